@@ -1,0 +1,71 @@
+// sehf_kernels.h -- launch wrappers for the gfx950 FP64 kernels.
+//
+// All matrices are FP64.  Unless stated otherwise they are column-major with
+// an explicit leading dimension, and "batched" means `batch` matrices at a
+// fixed element stride.  Every wrapper enqueues on the given stream and
+// returns without synchronising.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sehf {
+
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]   (column-major)
+// Columns of C in [zc0, zc1) take beta = 0 (pivot-block rows of the
+// Gauss-Jordan update, see gj_inverse).  A stride of 0 shares an operand
+// across the batch.
+struct Gemm {
+    int M = 0, N = 0, K = 0;
+    const double* A = nullptr; int64_t lda = 0, sA = 0; int tA = 0;
+    const double* B = nullptr; int64_t ldb = 0, sB = 0; int tB = 0;
+    double* C = nullptr; int64_t ldc = 0, sC = 0;
+    double alpha = 1.0, beta = 0.0;
+    int zc0 = 0, zc1 = 0;
+    int batch = 1;
+};
+void gemm(const Gemm& g, hipStream_t s);
+
+// In-place batched inverse by blocked Gauss-Jordan elimination with partial
+// pivoting.  `mat` holds `batch` column-major n x n matrices (leading dim ld,
+// element stride `stride`).  The inverse is written to `out` (may NOT alias
+// mat; same ld/stride), symmetrised as 1/2 (X + X^T) when `symmetrize`.
+// Workspace: piv  = batch * n ints, perm = batch * n ints,
+//            tbuf = batch * n * n doubles (block-row copies),
+//            info = batch ints (set to 1 + first zero-pivot index, else 0).
+struct GjWork {
+    int* piv = nullptr;
+    int* perm = nullptr;
+    double* tbuf = nullptr;
+    int* info = nullptr;
+};
+void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
+                bool symmetrize, const GjWork& w, hipStream_t s);
+size_t gj_tbuf_elems(int n);   // doubles per matrix needed in tbuf
+
+// Y[b] = a * X[b] + c * Z[b] + d * I   (n x m column-major blocks; Z may be null).
+void axpbyi(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX, double a,
+            const double* Z, int64_t ldz, int64_t sZ, double c, double d, int n, int m, int batch,
+            hipStream_t s);
+
+// Y[b] = M0 + shift[b] * I   (shared n x n source M0 with leading dim ldm).
+void shifted_copy(double* Y, int64_t ldy, int64_t sY, const double* M0, int64_t ldm,
+                  const double* shift, int n, int batch, hipStream_t s);
+
+// Transposed copy: Y[b](j, i) = a * X[b](i, j) for an r x c block X.
+void transpose_scaled(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx,
+                      int64_t sX, double a, int r, int c, int batch, hipStream_t s);
+
+// Column scale: Y(i, j) = X(i, j) * d[j]  (r x c, single matrix).
+void col_scale(double* Y, int64_t ldy, const double* X, int64_t ldx, const double* d, int r, int c,
+               hipStream_t s);
+
+// Fill n entries with a value (graph-capturable replacement for memset).
+void fill(double* p, size_t n, double v, hipStream_t s);
+void fill_int(int* p, size_t n, int v, hipStream_t s);
+
+// max_{b,i,j} |X[b](i,j) - X[b](j,i)| into *out (device double), n x n blocks.
+void max_asym(const double* X, int64_t ld, int64_t sX, int n, int batch, double* out, hipStream_t s);
+
+}  // namespace sehf
