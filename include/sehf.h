@@ -112,6 +112,55 @@ int  sehf_dme_create(sehf_ctx* ctx, int N_psi, int n_slots, sehf_dme** out);
 int  sehf_dme_step(sehf_dme* s, const double* V, const double* psi_ir, double* result_out);
 void sehf_dme_destroy(sehf_dme* s);
 
+/* ---- stage-level batched scan ----------------------------------------
+ * Replaces the per-ik loop of the scattering driver (src/main.cpp:847-877)
+ * and the stage chain inside run_one_energy (main.cpp:1081-1623):
+ * SchurInverter -> ForwardRPropagator -> KMatrixExtractor -> BackPropagator
+ * -> AsymptoticAmplitudes -> DipoleMatrixElement::compute_six.
+ * sehf_scan_create uploads the energy-independent inputs once (V(r), the
+ * exchange coupling, dipole tables); sehf_scan_run advances `batch` energies
+ * at a time through the radial grid (batch <= 0: all at once) with R
+ * checkpoints every `seg_len` points (<= 0: ceil(sqrt(Nr))).
+ * Per-energy outputs mirror DipoleEnergyPayload (src/scatt/DipoleIO.hpp:82-112);
+ * N = (l_cont+1)^2, slot order L x, L y, L z, V x, V y, V z. */
+typedef struct {
+    double* K;          /* [nk][N*N]  K from the two-point match (symmetrised)     */
+    double* A;          /* [nk][N*N]  fitted amplitude of j-hat                    */
+    double* B;          /* [nk][N*N]  fitted amplitude of y-hat                    */
+    double* K_fit;      /* [nk][N*N]  B A^{-1}                                     */
+    double* b_overlap;  /* [nk][n_occ][N]  int phi_a . psi_beta                    */
+    double* d_raw;      /* [nk][6][N]                                              */
+    double* D_raw;      /* [nk][6][N] complex interleaved (M d_raw)                */
+    double* D;          /* [nk][6][N] complex interleaved: the reduced dipole d_lm */
+    double* d_corr;     /* [6][n_occ] energy independent                           */
+    double* diag;       /* [nk][16]: fit_residual_rel, fit_residual_max,
+                           K_fit symmetry error, max|A - I|, n_transition, n_start,
+                           fit window lo, hi, A shifts, S shifts, closed l count  */
+} sehf_scan_out;
+
+int  sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out);
+/* info[8] = N_psi, N_f, N_total, n_transition, n_start, dipole points, Nr, l_exch */
+int  sehf_scan_info(const sehf_scan* s, int* info);
+int  sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out);
+void sehf_scan_destroy(sehf_scan* s);
+
+/* ---- host-only inputs (no device needed) -------------------------------
+ * The energy-independent matrices and scalar tables the scan uploads, for
+ * checking and for callers that stage their own inputs:
+ *   V(ir)  N x N, Potentials assembly (src/scatt/Potentials.cpp:324-365);
+ *   B(ir)  N x N_f = h^2/12 Q_psi_f (src/scatt/ExchangeCoupling.cpp:66-136);
+ *   dipole rows (n_pts x (6+n_occ) x N, Simpson-weighted) and d_corr (6 x n_occ);
+ *   sizes: N_psi, N_f, N_total, n_transition, n_start, Nr, l_exch, n_lambda;
+ *   Riccati-Bessel / -Neumann, real Gaunt, smallest symmetric eigenvalue. */
+int    sehf_host_potential(const sehf_problem* p, int ir, double* V);
+int    sehf_host_exchange_B(const sehf_problem* p, int ir, double* B);
+int    sehf_host_sizes(const sehf_problem* p, int* info);
+int    sehf_host_dipole_tables(const sehf_problem* p, double* rows, double* dcorr, int* n_pts);
+double sehf_host_ric_j(int l, double x);
+double sehf_host_ric_y(int l, double x);
+double sehf_host_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3);
+double sehf_host_min_eig(int n, const double* a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,17 @@ def solve_scan(p, ks, threads=None, **kw):
         return list(ex.map(lambda k: solve(p, float(k), **kw), ks))
 
 
+def noise_floor(p, ks, **kw):
+    """Oracle run with Gauss-Jordan elimination in place of LU: the spread to
+    the LU run measures the reference algorithm's own rounding noise."""
+    L = lib()
+    L.sehf_oracle_set_inverse_mode(1)
+    try:
+        return solve_scan(p, ks, **kw)
+    finally:
+        L.sehf_oracle_set_inverse_mode(0)
+
+
 def time_steps(p, k, n_steps):
     L = lib()
     ref = ProblemRef(p)

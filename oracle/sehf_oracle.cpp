@@ -70,21 +70,27 @@ static Mat transpose(const Mat& A) {
     for (int j = 0; j < A.c; ++j) for (int i = 0; i < A.r; ++i) T(j, i) = A(i, j);
     return T;
 }
-// C = alpha*A*B + beta*C  (column-major, j-p-i loop order for contiguous inner loop)
+// C = alpha*(A*B) + beta*C  (column-major).  BLAS/Eigen semantics: the
+// product is formed first and then combined with C, so an update such as
+// S = A - Bs*B^T groups as A_ij - (sum_p ...), the way the reference's
+// GEMM evaluates it (SchurInverter.cpp:347-351).
 static void gemm(double alpha, const Mat& A, const Mat& B, double beta, Mat& C) {
     const int m = A.r, k = A.c, n = B.c;
     if (C.r != m || C.c != n) { C.resize(m, n); beta = 0.0; }
+    std::vector<double> t(m);
     for (int j = 0; j < n; ++j) {
         double* cj = C.col(j);
-        if (beta == 0.0) std::fill(cj, cj + m, 0.0);
-        else if (beta != 1.0) for (int i = 0; i < m; ++i) cj[i] *= beta;
+        std::fill(t.begin(), t.end(), 0.0);
         const double* bj = B.col(j);
         for (int p = 0; p < k; ++p) {
-            const double s = alpha * bj[p];
+            const double s = bj[p];
             if (s == 0.0) continue;
             const double* ap = A.col(p);
-            for (int i = 0; i < m; ++i) cj[i] += s * ap[i];
+            for (int i = 0; i < m; ++i) t[i] += s * ap[i];
         }
+        if (beta == 0.0) for (int i = 0; i < m; ++i) cj[i] = alpha * t[i];
+        else if (beta == 1.0) for (int i = 0; i < m; ++i) cj[i] = cj[i] + alpha * t[i];
+        else for (int i = 0; i < m; ++i) cj[i] = beta * cj[i] + alpha * t[i];
     }
 }
 static Mat mul(const Mat& A, const Mat& B) { Mat C(A.r, B.c); gemm(1.0, A, B, 0.0, C); return C; }
@@ -159,9 +165,37 @@ static MatT<T> inverse_pp(const MatT<T>& A) {
     lu_solve(lu, perm, X);
     return X;
 }
-static Mat inverse_general(const Mat& A) { return inverse_pp(A); }
+// Gauss-Jordan elimination with partial pivoting: an equally valid
+// elimination order, used only to measure how far two correct rounding
+// orders of the reference's algorithm drift apart (the parity noise floor).
+static Mat inverse_gj(const Mat& A) {
+    const int n = A.r;
+    Mat a = A;
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k; double best = std::abs(a(k, k));
+        for (int i = k + 1; i < n; ++i) if (std::abs(a(i, k)) > best) { best = std::abs(a(i, k)); p = i; }
+        piv[k] = p;
+        if (p != k) for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
+        const double pinv = 1.0 / a(k, k);
+        for (int j = 0; j < n; ++j) if (j != k) a(k, j) *= pinv;
+        a(k, k) = pinv;
+        for (int j = 0; j < n; ++j) {
+            if (j == k) continue;
+            const double f = a(k, j);
+            double* cj = a.col(j);
+            const double* ck = a.col(k);
+            for (int i = 0; i < n; ++i) if (i != k) cj[i] -= ck[i] * f;
+        }
+        for (int i = 0; i < n; ++i) if (i != k) a(i, k) = -a(i, k) * pinv;
+    }
+    for (int k = n - 1; k >= 0; --k) if (piv[k] != k) for (int i = 0; i < n; ++i) std::swap(a(i, k), a(i, piv[k]));
+    return a;
+}
+static int g_inverse_mode = 0;   // 0: LU (the reference), 1: Gauss-Jordan (noise-floor probe)
+static Mat inverse_general(const Mat& A) { return g_inverse_mode == 1 ? inverse_gj(A) : inverse_pp(A); }
 // inverse_symmetric_indefinite, no-MKL branch (LapackInverse.hpp:174-176).
-static Mat inverse_symmetric_indefinite(const Mat& A) { Mat X = inverse_pp(A); symmetrize(X); return X; }
+static Mat inverse_symmetric_indefinite(const Mat& A) { Mat X = inverse_general(A); symmetrize(X); return X; }
 
 // Full-pivot LU inverse (Eigen FullPivLU; AsymptoticAmplitudes.cpp:317-322).
 static bool inverse_fullpiv(const Mat& A, Mat& Xout) {
@@ -1240,6 +1274,8 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
 extern "C" {
 
 const char* sehf_oracle_last_error(void) { return orc::g_err.c_str(); }
+
+void sehf_oracle_set_inverse_mode(int mode) { orc::g_inverse_mode = mode; }
 
 int sehf_oracle_solve(const sehf_oracle_problem* p, double k, sehf_oracle_out* out) {
     try {

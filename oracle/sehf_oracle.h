@@ -72,6 +72,13 @@ int sehf_oracle_time_steps(const sehf_oracle_problem* p, double k, int n_steps, 
 
 const char* sehf_oracle_last_error(void);
 
+/* Dense-inverse elimination order: 0 = partial-pivot LU as the reference
+ * (default); 1 = Gauss-Jordan with partial pivoting.  Mode 1 is a second,
+ * equally valid rounding order, used by tests only to measure the rounding
+ * noise floor of the reference algorithm.  Process-wide; not thread-safe
+ * against concurrent solves. */
+void sehf_oracle_set_inverse_mode(int mode);
+
 /* ---- primitives exposed for golden-vector tests ---- */
 double sehf_oracle_w3j(int j1, int j2, int j3, int m1, int m2, int m3);
 double sehf_oracle_gaunt_real(int l1, int m1, int l2, int m2, int l3, int m3);

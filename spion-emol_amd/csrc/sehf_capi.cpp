@@ -14,6 +14,7 @@
 namespace sehf {
 
 static thread_local std::string g_last_error;
+std::string& last_error_ref() { return g_last_error; }
 
 void check_gj_info(const GjBuffers& gb, int batch, hipStream_t s, const char* what) {
     std::vector<int> info(batch);
@@ -50,6 +51,8 @@ struct sehf_ctx {
     size_t mem = 0;
     int is_gpu = 1;
 };
+
+hipStream_t sehf_ctx_stream(const sehf_ctx* c) { return c->stream; }
 
 struct sehf_frp {
     sehf_ctx* ctx;
@@ -106,7 +109,7 @@ int sehf_ctx_create(int prefer_gpu, sehf_ctx** out) {
         SEHF_HIP_CHECK(hipGetDevice(&c->device));
         hipDeviceProp_t prop;
         SEHF_HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
-        c->name = prop.name;
+        c->name = prop.name[0] ? std::string(prop.name) : std::string(prop.gcnArchName);
         c->mem = prop.totalGlobalMem;
         SEHF_HIP_CHECK(hipStreamCreate(&c->stream));
         *out = c;
@@ -158,6 +161,7 @@ int sehf_frp_step(sehf_frp* s, const double* W_inv, double* Rinv_out, double* sy
         SEHF_HIP_CHECK(hipMemcpyAsync(s->W.p, W_inv, nn * sizeof(double), hipMemcpyHostToDevice, st));
         // U = 12 W_inv - 10 I - Rinv_{n-1}
         axpbyi(s->U.p, N, 0, s->W.p, N, 0, 12.0, s->R.p, N, 0, -1.0, -10.0, N, N, 1, st);
+        fill_int(s->gj.info.p, 1, 0, st);
         gj_inverse(s->U.p, s->R.p, N, N, 0, 1, true, s->gj.work(), st);
         check_gj_info(s->gj, 1, st, "sehf_frp_step");
         if (Rinv_out)
@@ -187,6 +191,7 @@ int sehf_sinv_create(sehf_ctx* ctx, int N_psi, int N_f, sehf_sinv** out) {
 static void sinv_common(sehf_sinv* s, double* Sinv_out, double* sym_err) {
     hipStream_t st = s->ctx->stream;
     const int Np = s->Np;
+    fill_int(s->gj.info.p, 1, 0, st);
     gj_inverse(s->A.p, s->Sinv.p, Np, Np, 0, 1, true, s->gj.work(), st);
     check_gj_info(s->gj, 1, st, "sehf_sinv_step");
     if (Sinv_out)

@@ -60,6 +60,8 @@ struct GjBuffers {
         perm.alloc(static_cast<size_t>(n) * batch);
         info.alloc(static_cast<size_t>(batch));
         tbuf.alloc(gj_tbuf_elems(n) * batch);
+        fill_int(info.p, static_cast<size_t>(batch), 0, nullptr);
+        SEHF_HIP_CHECK(hipDeviceSynchronize());
     }
     GjWork work() const { return GjWork{piv.p, perm.p, tbuf.p, info.p}; }
 };

@@ -33,7 +33,8 @@ void gemm(const Gemm& g, hipStream_t s);
 // mat; same ld/stride), symmetrised as 1/2 (X + X^T) when `symmetrize`.
 // Workspace: piv  = batch * n ints, perm = batch * n ints,
 //            tbuf = batch * n * n doubles (block-row copies),
-//            info = batch ints (set to 1 + first zero-pivot index, else 0).
+//            info = batch ints: a zero pivot at index k sets 1 + k if still 0
+//            (sticky: the caller clears it, so one check covers many calls).
 struct GjWork {
     int* piv = nullptr;
     int* perm = nullptr;

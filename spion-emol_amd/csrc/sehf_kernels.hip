@@ -359,7 +359,6 @@ static void gj_rec(double* mat, int n, int64_t ld, int64_t stride, int batch, in
 void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
                 bool symmetrize, const GjWork& W, hipStream_t s) {
     if (n <= 0 || batch <= 0) return;
-    fill_int(W.info, batch, 0, s);
     gj_rec(mat, n, ld, stride, batch, 0, n, W, s);
     hipLaunchKernelGGL(k_gj_perm, dim3(batch), dim3(256), sizeof(int) * n, s, W.piv, W.perm, n);
     dim3 fg((n + PT - 1) / PT, (n + PT - 1) / PT, batch);

@@ -1,0 +1,681 @@
+// sehf_scan.cpp -- the batched energy scan on one device.
+//
+// All energies of a batch advance through the radial grid together.  Per
+// radial step n and energy b (SchurInverter / ForwardRPropagator,
+// src/scatt/SchurInverter.cpp:247-415, ForwardRPropagator.cpp:299-386):
+//   Sinv_b = sym((S0_n + shift_b(n) I)^{-1}),   S0_n = I - h^2/6 V_n - B_n D_n^{-1} B_n^T
+//   X_b    = Sinv_b (B_n D_n^{-1})
+//   U_b    = 12 W_n^{-1} - 10 I - R_{n-1,b}   (W^{-1} from Sinv_b, X_b, D^{-1})
+//   R_n,b  = sym(U_b^{-1})
+// Only sqrt(Nr)-spaced R checkpoints are kept.  The back-propagation
+// (BackPropagator.cpp:603-703) walks the grid downwards one segment at a
+// time, recomputing that segment's Sinv and R from its checkpoint, and folds
+// the dipole integrands (DipoleMatrixElement.cpp:584-965) and the asymptotic
+// fit sums (AsymptoticAmplitudes.cpp:64-359) into the sweep, so psi is never
+// stored except for the ~200-point fit window.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sehf.h"
+#include "sehf_common.h"
+#include "sehf_host_math.h"
+#include "sehf_scan_kernels.h"
+#include "sehf_setup.h"
+
+namespace sehf {
+std::string& last_error_ref();
+}
+
+using namespace sehf;
+
+namespace {
+
+struct Batch {
+    int B = 0;   // capacity
+    DevArray<double> Rp, Rc, U, Ms, Sinv, X, Z, Zt, T, Psi, acc, YJ, YY, Am, Bm, Kd, P, Vpsi, G, Bf;
+    DevArray<double> ckpt, segR, segS, win, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
+    DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
+    GjBuffers gj;
+};
+
+}  // namespace
+
+struct sehf_scan {
+    sehf_ctx* ctx = nullptr;
+    hipStream_t st = nullptr;
+    std::unique_ptr<Setup> S;
+    int Np = 0, Nf = 0, Nt = 0, Nr = 0, n_tr = 0, n_start = 1, n_pts = 0, nslot = 0, n_bd = 0, nstab = 0;
+    double h = 0, c6 = 0;
+    DevArray<double> S0, BD, Dinv, VN, BN, negDinvN, Drows, dcorr, cvd, bdb;
+    std::vector<double> dcorr_h, vdiag;       // vdiag: (n_start+2) x Np potential diagonal
+    std::vector<double> gA, gS, lA, lS;
+    std::vector<char> hasLA, hasLS;
+    std::vector<std::vector<double>> A0h, S0h;
+    Batch bt;
+    size_t dev_bytes_shared = 0;
+};
+
+// context stream accessor (sehf_ctx is defined in sehf_capi.cpp)
+hipStream_t sehf_ctx_stream(const sehf_ctx* c);
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        last_error_ref() = e.what();
+        return 1;
+    } catch (...) {
+        last_error_ref() = "unknown error";
+        return 1;
+    }
+}
+
+void build_shared(sehf_scan* s) {
+    Setup& S = *s->S;
+    hipStream_t st = s->st;
+    const int Np = s->Np, Nf = s->Nf, Nr = s->Nr;
+    const size_t pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * Nf;
+    s->S0.alloc(pp * Nr);
+    s->cvd.alloc(static_cast<size_t>(Np) * Nr);
+    s->bdb.alloc(static_cast<size_t>(Np) * Nr);
+    fill(s->bdb.p, static_cast<size_t>(Np) * Nr, 0.0, st);
+    std::vector<double> cvd_h(static_cast<size_t>(Np) * Nr);
+    s->n_bd = std::min(s->n_tr, Nr);
+    if (Nf > 0) s->BD.alloc(pf * s->n_bd);
+    s->Dinv.alloc(static_cast<size_t>(std::max(Nf, 1)) * Nr);
+    {
+        std::vector<double> dh(static_cast<size_t>(std::max(Nf, 1)) * Nr, 0.0);
+        for (int n = 0; n < Nr; ++n) S.dinv(n, dh.data() + static_cast<size_t>(n) * std::max(Nf, 1));
+        SEHF_HIP_CHECK(hipMemcpyAsync(s->Dinv.p, dh.data(), dh.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    s->nstab = std::min(Setup::stab_n_max, Nr);
+    s->A0h.assign(s->nstab, {});
+    s->S0h.assign(s->nstab, {});
+    const int chunk = 32;
+    std::vector<double> Vh(pp * chunk), Bh(std::max<size_t>(pf, 1) * chunk);
+    DevArray<double> Vd(pp * chunk), Bd(std::max<size_t>(pf, 1) * chunk), Bsc(std::max<size_t>(pf, 1) * chunk);
+    for (int n0 = 0; n0 < Nr; n0 += chunk) {
+        const int nc = std::min(chunk, Nr - n0);
+        S.build_chunk(n0, nc, Vh.data(), Bh.data());
+        for (int q = 0; q < nc; ++q)
+            for (int i = 0; i < Np; ++i)
+                cvd_h[static_cast<size_t>(n0 + q) * Np + i] = s->c6 * Vh[q * pp + i + static_cast<size_t>(i) * Np];
+        SEHF_HIP_CHECK(hipMemcpyAsync(Vd.p, Vh.data(), nc * pp * sizeof(double), hipMemcpyHostToDevice, st));
+        double* S0c = s->S0.p + static_cast<size_t>(n0) * pp;
+        // S0 = I - h^2/6 V
+        axpbyi(S0c, Np, pp, Vd.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, nc, st);
+        const int nb = std::max(0, std::min(nc, s->n_bd - n0));
+        if (Nf > 0 && nb > 0) {
+            SEHF_HIP_CHECK(hipMemcpyAsync(Bd.p, Bh.data(), nb * pf * sizeof(double), hipMemcpyHostToDevice, st));
+            for (int q = 0; q < nb; ++q)
+                col_scale(s->BD.p + static_cast<size_t>(n0 + q) * pf, Np, Bd.p + q * pf, Np,
+                          s->Dinv.p + static_cast<size_t>(n0 + q) * Nf, Np, Nf, st);
+            // (B D^{-1} B^T)_ii as a separate dot product for the per-energy diagonal
+            row_dot(s->bdb.p + static_cast<size_t>(n0) * Np, s->BD.p + static_cast<size_t>(n0) * pf, Bd.p, Np, Nf, nb, st);
+            Gemm g;   // S0 -= (B D^{-1}) B^T
+            g.M = Np; g.N = Np; g.K = Nf;
+            g.A = s->BD.p + static_cast<size_t>(n0) * pf; g.lda = Np; g.sA = pf;
+            g.B = Bd.p; g.ldb = Np; g.sB = pf; g.tB = 1;
+            g.C = S0c; g.ldc = Np; g.sC = pp;
+            g.alpha = -1.0; g.beta = 1.0; g.batch = nb;
+            gemm(g, st);
+        }
+        // host copies of the small-r matrices that may need the stability shift
+        for (int q = 0; q < nc && n0 + q < s->nstab; ++q) {
+            const int n = n0 + q;
+            auto& A0 = s->A0h[n];
+            A0.resize(pp);
+            for (size_t t = 0; t < pp; ++t) A0[t] = -s->c6 * Vh[q * pp + t];
+            for (int i = 0; i < Np; ++i) A0[i + static_cast<size_t>(i) * Np] += 1.0;
+            s->S0h[n].resize(pp);
+            SEHF_HIP_CHECK(hipMemcpyAsync(s->S0h[n].data(), S0c + q * pp, pp * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        if (n0 + nc >= Nr) {   // boundary n = Nr-1: V_N and unscaled B_N
+            const int q = Nr - 1 - n0;
+            s->VN.alloc(pp);
+            SEHF_HIP_CHECK(hipMemcpyAsync(s->VN.p, Vh.data() + q * pp, pp * sizeof(double), hipMemcpyHostToDevice, st));
+            if (Nf > 0) {
+                s->BN.alloc(pf);
+                SEHF_HIP_CHECK(hipMemcpyAsync(s->BN.p, Bh.data() + q * pf, pf * sizeof(double), hipMemcpyHostToDevice, st));
+            }
+        }
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    SEHF_HIP_CHECK(hipMemcpyAsync(s->cvd.p, cvd_h.data(), cvd_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    // Gershgorin bounds of A0_n and S0_n (SchurInverter.cpp:30-49)
+    auto gersh = [Np](const std::vector<double>& M) {
+        double lb = 1e300;
+        for (int i = 0; i < Np; ++i) {
+            double off = 0.0;
+            for (int j = 0; j < Np; ++j) if (j != i) off += std::fabs(M[i + static_cast<size_t>(j) * Np]);
+            lb = std::min(lb, M[i + static_cast<size_t>(i) * Np] - off);
+        }
+        return lb;
+    };
+    s->gA.assign(s->nstab, 0.0); s->gS.assign(s->nstab, 0.0);
+    s->lA.assign(s->nstab, 0.0); s->lS.assign(s->nstab, 0.0);
+    s->hasLA.assign(s->nstab, 0); s->hasLS.assign(s->nstab, 0);
+    for (int n = 0; n < s->nstab; ++n) { s->gA[n] = gersh(s->A0h[n]); s->gS[n] = gersh(s->S0h[n]); }
+    if (Nf > 0) {
+        std::vector<double> nd(Nf);
+        S.dinv(Nr - 1, nd.data());
+        for (double& v : nd) v = -v;
+        s->negDinvN.alloc(Nf);
+        SEHF_HIP_CHECK(hipMemcpyAsync(s->negDinvN.p, nd.data(), Nf * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    // dipole rows and orthogonalisation integrals
+    std::vector<double> rows;
+    S.dipole_tables(rows, s->dcorr_h, s->n_pts);
+    s->Drows.alloc(rows.size());
+    SEHF_HIP_CHECK(hipMemcpyAsync(s->Drows.p, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    s->dcorr.alloc(std::max<size_t>(s->dcorr_h.size(), 1));
+    SEHF_HIP_CHECK(hipMemcpyAsync(s->dcorr.p, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    // potential diagonal for the analytic start (ForwardRPropagator.cpp:78-134)
+    s->vdiag.assign(static_cast<size_t>(s->n_start + 2) * Np, 0.0);
+    {
+        std::vector<double> V(pp);
+        for (int n = 0; n <= std::min(s->n_start + 1, Nr - 1); ++n) {
+            S.potential(n, V.data());
+            for (int i = 0; i < Np; ++i) s->vdiag[static_cast<size_t>(n) * Np + i] = V[i + static_cast<size_t>(i) * Np];
+        }
+    }
+    SEHF_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+double lmin_A(sehf_scan* s, int n) {
+    if (!s->hasLA[n]) { s->lA[n] = sym_min_eig(s->Np, s->A0h[n].data()); s->hasLA[n] = 1; }
+    return s->lA[n];
+}
+double lmin_S(sehf_scan* s, int n) {
+    if (!s->hasLS[n]) { s->lS[n] = sym_min_eig(s->Np, s->S0h[n].data()); s->hasLS[n] = 1; }
+    return s->lS[n];
+}
+
+// Diagonal shift of S0_n for energy E (SchurInverter.cpp:254-372): h^2/6 E,
+// plus the Johnson stability shifts of A and S for n < 60.
+void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_out, double& dS_out) {
+    const double cE = s->c6 * E;
+    dA_out = dS_out = 0.0;
+    if (n >= s->nstab) return;
+    const double thr = Setup::W_min;
+    double meA = s->gA[n] + cE;
+    if (!(meA > thr + 0.05)) meA = lmin_A(s, n) + cE;
+    double dA = 0.0;
+    if (meA < thr) { dA = thr - meA + Setup::shift_margin; ++shA; }
+    dA_out = dA;
+    if (n >= s->n_tr || s->Nf == 0) return;
+    double meS = s->gS[n] + cE + dA;
+    if (!(meS > thr + 0.05)) meS = lmin_S(s, n) + cE + dA;
+    if (meS < thr) { dS_out = thr - meS + Setup::shift_margin; ++shS; }
+}
+
+void alloc_batch(sehf_scan* s, int B, int L, int n_seg, int n_fit) {
+    Batch& b = s->bt;
+    if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_seg, 1)) * B * s->Nt * s->Nt &&
+        b.win.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np * s->Np)
+        return;
+    const size_t Np = s->Np, Nf = s->Nf, Nt = s->Nt, ns = s->nslot;
+    const size_t tt = Nt * Nt, pp = Np * Np, pf = Np * std::max<size_t>(Nf, 1), ff = std::max<size_t>(Nf, 1) * std::max<size_t>(Nf, 1);
+    b = Batch();
+    b.B = B;
+    b.Rp.alloc(tt * B); b.Rc.alloc(tt * B); b.U.alloc(tt * B);
+    b.Ms.alloc(pp * B); b.Sinv.alloc(pp * B); b.X.alloc(pf * B);
+    b.Z.alloc(Nt * Np * B); b.Zt.alloc(Nt * Np * B);
+    b.T.alloc(pp * B); b.Psi.alloc(pp * B); b.acc.alloc(ns * Np * B);
+    b.YJ.alloc(pp * B); b.YY.alloc(pp * B); b.Am.alloc(pp * B); b.Bm.alloc(pp * B); b.Kd.alloc(pp * B);
+    b.P.alloc(pp * B); b.Vpsi.alloc(pp * B); b.G.alloc(pf * B); b.Bf.alloc(pp * B);
+    b.ckpt.alloc(tt * B * std::max(n_seg, 1));
+    b.segR.alloc(tt * B * L); b.segS.alloc(pp * B * L);
+    b.win.alloc(pp * B * std::max(n_fit, 1));
+    b.W2.alloc(4 * pp * B); b.W2i.alloc(4 * pp * B);
+    b.Rff.alloc(ff * B); b.Rffi.alloc(ff * B); b.T1.alloc(pf * B);
+    b.DRre.alloc(6 * Np * B); b.DRim.alloc(6 * Np * B); b.ore.alloc(Np * s->S->n_occ * B); b.oim.alloc(Np * s->S->n_occ * B);
+    b.Dre.alloc(6 * Np * B); b.Dim.alloc(6 * Np * B); b.Ainv.alloc(pp * B); b.Kfit.alloc(pp * B);
+    b.dA.alloc(static_cast<size_t>(s->nstab + 1) * B); b.dS.alloc(static_cast<size_t>(s->nstab + 1) * B);
+    b.zeros.alloc(B); fill(b.zeros.p, B, 0.0, s->st); b.cE.alloc(B);
+    b.anl.alloc(static_cast<size_t>(s->n_start + 1) * B * Nt);
+    b.tab.alloc(6 * Np * B); b.jy.alloc(2 * Np * B);
+    b.Jc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.Yc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
+    b.coef.alloc(5 * Np * B); b.red.alloc(2 * B); b.nrm.alloc(2 * B);
+    b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
+}
+
+void up(double* d, const std::vector<double>& h, hipStream_t st) {
+    if (!h.empty()) SEHF_HIP_CHECK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+}
+
+void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out, int out_off) {
+    Setup& S = *s->S;
+    hipStream_t st = s->st;
+    Batch& b = s->bt;
+    const int Np = s->Np, Nf = s->Nf, Nt = s->Nt, Nr = s->Nr, N = Nr - 1, ns = s->nslot, n_occ = S.n_occ;
+    const size_t tt = static_cast<size_t>(Nt) * Nt, pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * std::max(Nf, 1);
+    const int n_seg = (N + L - 1) / L;
+    const double h = s->h;
+    // ---------------- host tables for this batch ----------------
+    std::vector<double> E(nb), cE(nb);
+    for (int e = 0; e < nb; ++e) { E[e] = 0.5 * ks[e] * ks[e]; cE[e] = s->c6 * E[e]; }
+    std::vector<int> shA(nb, 0), shS(nb, 0);
+    std::vector<double> dA(static_cast<size_t>(s->nstab) * nb), dS(dA.size());
+    for (int n = 0; n < s->nstab; ++n)
+        for (int e = 0; e < nb; ++e)
+            sinv_shift(s, n, E[e], shA[e], shS[e], dA[static_cast<size_t>(n) * nb + e], dS[static_cast<size_t>(n) * nb + e]);
+    // analytic diagonal R for n in [0, n_start)  (ForwardRPropagator.cpp:78-134)
+    const int ns0 = s->n_start;
+    std::vector<double> anl(static_cast<size_t>(ns0) * nb * Nt, 0.0);
+    for (int n = 1; n < ns0; ++n)
+        for (int e = 0; e < nb; ++e) {
+            double* d = anl.data() + (static_cast<size_t>(n) * nb + e) * Nt;
+            const double k = ks[e], h2_12 = h * h / 12.0, h2_6 = h * h / 6.0;
+            const double rn = h + n * h, rn1 = h + (n + 1) * h;
+            for (int mu = 0; mu < Np; ++mu) {
+                const double Mn = 1.0 + h2_6 * (E[e] - s->vdiag[static_cast<size_t>(n) * Np + mu]);
+                const double Mn1 = 1.0 + h2_6 * (E[e] - s->vdiag[static_cast<size_t>(n + 1) * Np + mu]);
+                const double den = Mn1 * ric_j(S.l_mu[mu], k * rn1);
+                if (std::fabs(den) > 1e-300) d[mu] = (Mn * ric_j(S.l_mu[mu], k * rn)) / den;
+            }
+            for (int f = 0; f < Nf; ++f) {
+                const int l = S.l_sig[f % S.n_sigma];
+                double Mn = 1.0 - h2_12 * (double(l * (l + 1)) / (rn * rn));
+                double Mn1 = 1.0 - h2_12 * (double(l * (l + 1)) / (rn1 * rn1));
+                if (Mn < Setup::W_min) Mn = Setup::W_min;
+                if (Mn1 < Setup::W_min) Mn1 = Setup::W_min;
+                const double den = Mn1 * std::pow(rn1, static_cast<double>(l + 1));
+                if (std::fabs(den) > 1e-300) d[Np + f] = (Mn * std::pow(rn, static_cast<double>(l + 1))) / den;
+            }
+        }
+    // K-matrix matching tables (KMatrixExtractor.cpp:126-144), psi_N tables
+    std::vector<double> tab(static_cast<size_t>(6) * Np * nb), jy(static_cast<size_t>(2) * Np * nb);
+    const double r_in = h + (Nr - 2) * h, r_out = h + (Nr - 1) * h, rN = h + (Nr - 1) * h;
+    for (int e = 0; e < nb; ++e) {
+        const double k = ks[e], k2 = k * k, h2_12 = h * h / 12.0;
+        double* t = tab.data() + static_cast<size_t>(e) * 6 * Np;
+        double* q = jy.data() + static_cast<size_t>(e) * 2 * Np;
+        for (int mu = 0; mu < Np; ++mu) {
+            const int l = S.l_mu[mu];
+            t[mu] = ric_j(l, k * r_in); t[Np + mu] = ric_j(l, k * r_out);
+            t[2 * Np + mu] = ric_y(l, k * r_in); t[3 * Np + mu] = ric_y(l, k * r_out);
+            t[4 * Np + mu] = 1.0 + h2_12 * (k2 - static_cast<double>(l * (l + 1)) / (r_in * r_in));
+            t[5 * Np + mu] = 1.0 + h2_12 * (k2 - static_cast<double>(l * (l + 1)) / (r_out * r_out));
+            q[mu] = ric_j(l, k * rN); q[Np + mu] = ric_y(l, k * rN);
+        }
+    }
+    // fit window (AsymptoticAmplitudes.cpp:70-91) clamped to the 300-point tail
+    const int asym_off = std::max(0, N - 300 + 1);
+    int n_fs = std::max(Nr - 200, static_cast<int>(0.9 * Nr));
+    int n_fe = Nr - 1 - 3;
+    n_fs = std::max(n_fs, asym_off);
+    n_fe = std::min(n_fe, asym_off + 300 - 1);
+    if (n_fe - n_fs + 1 < 10) throw std::runtime_error("asymptotic fit window is too small");
+    const int n_fit = n_fe - n_fs + 1;
+    std::vector<double> Jc(static_cast<size_t>(n_fit) * Np * nb), Yc(Jc.size()), coef(static_cast<size_t>(5) * Np * nb);
+    std::vector<int> n_closed(nb, 0);
+    {
+        int lmax = 0;
+        for (int mu = 0; mu < Np; ++mu) lmax = std::max(lmax, S.l_mu[mu]);
+        std::vector<double> jl(static_cast<size_t>(n_fit) * (lmax + 1)), yl(jl.size());
+        for (int e = 0; e < nb; ++e) {
+            const double k = ks[e];
+            for (int l = 0; l <= lmax; ++l)
+                for (int i = 0; i < n_fit; ++i) {
+                    const double r = h + (n_fs + i) * h;
+                    jl[static_cast<size_t>(l) * n_fit + i] = ric_j(l, k * r);
+                    yl[static_cast<size_t>(l) * n_fit + i] = ric_y(l, k * r);
+                }
+            std::vector<double> m(5 * (lmax + 1));
+            for (int l = 0; l <= lmax; ++l) {
+                double a = 0, bb = 0, c = 0;
+                for (int i = 0; i < n_fit; ++i) {
+                    const double x = jl[static_cast<size_t>(l) * n_fit + i], y = yl[static_cast<size_t>(l) * n_fit + i];
+                    a += x * x; bb += x * y; c += y * y;
+                }
+                const double det = a * c - bb * bb;
+                const double mx = std::max(std::fabs(a), std::max(std::fabs(bb), std::fabs(c)));
+                if (!(std::fabs(det) > 1e-20 * mx * mx)) {
+                    if (!S.P.allow_closed)
+                        throw std::runtime_error("asymptotic fit: singular normal matrix for l=" + std::to_string(l) +
+                                                 " (closed channel; enable allow_closed)");
+                    m[5 * l + 4] = 1.0;
+                    ++n_closed[e];
+                } else {
+                    const double id = 1.0 / det;
+                    m[5 * l + 0] = c * id; m[5 * l + 1] = -bb * id; m[5 * l + 2] = -bb * id; m[5 * l + 3] = a * id;
+                }
+            }
+            double* cf = coef.data() + static_cast<size_t>(e) * 5 * Np;
+            for (int mu = 0; mu < Np; ++mu) {
+                const int l = S.l_mu[mu];
+                for (int q = 0; q < 5; ++q) cf[q * Np + mu] = m[5 * l + q];
+                for (int i = 0; i < n_fit; ++i) {
+                    Jc[(static_cast<size_t>(e) * n_fit + i) * Np + mu] = jl[static_cast<size_t>(l) * n_fit + i];
+                    Yc[(static_cast<size_t>(e) * n_fit + i) * Np + mu] = yl[static_cast<size_t>(l) * n_fit + i];
+                }
+            }
+        }
+    }
+    up(b.dA.p, dA, st); up(b.dS.p, dS, st); up(b.cE.p, cE, st); up(b.anl.p, anl, st);
+    up(b.tab.p, tab, st); up(b.jy.p, jy, st); up(b.Jc.p, Jc, st); up(b.Yc.p, Yc, st); up(b.coef.p, coef, st);
+    const int64_t sJc = static_cast<int64_t>(n_fit) * Np, sWin = static_cast<int64_t>(n_fit) * pp;
+
+    auto gj = [&](double* mat, double* outp, int n, bool sym) {
+        gj_inverse(mat, outp, n, n, static_cast<int64_t>(n) * n, nb, sym, b.gj.work(), st);
+    };
+    auto sinv_step = [&](int n, double* Sout) {
+        const double* a = (n < s->nstab) ? b.dA.p + static_cast<size_t>(n) * nb : b.zeros.p;
+        const double* d = (n < s->nstab) ? b.dS.p + static_cast<size_t>(n) * nb : b.zeros.p;
+        sinv_matrix(b.Ms.p, s->S0.p + static_cast<size_t>(n) * pp, s->cvd.p + static_cast<size_t>(n) * Np,
+                    s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st);
+        gj(b.Ms.p, Sout, Np, true);
+    };
+    auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout) {
+        init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
+        if (n < s->n_tr && Nf > 0) {
+            Gemm g;   // X = Sinv (B D^{-1})
+            g.M = Np; g.N = Nf; g.K = Np;
+            g.A = Sv; g.lda = Np; g.sA = pp;
+            g.B = s->BD.p + static_cast<size_t>(n) * pf; g.ldb = Np; g.sB = 0;
+            g.C = b.X.p; g.ldc = Np; g.sC = pf; g.batch = nb;
+            gemm(g, st);
+            add_psi_f_blocks(b.U.p, b.X.p, -12.0, Np, Nf, Nt, nb, st);
+            Gemm h2;  // U_ff += 12 (B D^{-1})^T X
+            h2.M = Nf; h2.N = Nf; h2.K = Np;
+            h2.A = s->BD.p + static_cast<size_t>(n) * pf; h2.lda = Np; h2.sA = 0; h2.tA = 1;
+            h2.B = b.X.p; h2.ldb = Np; h2.sB = pf;
+            h2.C = b.U.p + Np + static_cast<size_t>(Np) * Nt; h2.ldc = Nt; h2.sC = tt;
+            h2.alpha = 12.0; h2.beta = 1.0; h2.batch = nb;
+            gemm(h2, st);
+        }
+        gj(b.U.p, Rout, Nt, true);
+    };
+    // ---------------- forward sweep ----------------
+    if (ns0 >= 2) diag_matrix(b.Rp.p, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
+    else fill(b.Rp.p, tt * nb, 0.0, st);
+    for (int n = ns0; n <= Nr - 2; ++n) {
+        sinv_step(n, b.Sinv.p);
+        frp_step(n, b.Sinv.p, b.Rp.p, b.Rc.p);
+        std::swap(b.Rp.p, b.Rc.p);
+        if ((n + 1) % L == 0) {
+            const int sidx = (n + 1) / L;
+            if (sidx < n_seg)
+                SEHF_HIP_CHECK(hipMemcpyAsync(b.ckpt.p + static_cast<size_t>(sidx) * nb * tt, b.Rp.p, nb * tt * sizeof(double),
+                                              hipMemcpyDeviceToDevice, st));
+        }
+    }
+    check_gj_info(b.gj, nb, st, "scan forward sweep");
+    // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
+    SEHF_HIP_CHECK(hipMemcpyAsync(b.Rc.p, b.Rp.p, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+    gj(b.Rc.p, b.U.p, Nt, false);   // U <- R = Rinv_{N-1}^{-1}
+    get_block(b.Am.p, b.U.p, Nt, tt, 0, 0, Np, Np, nb, st);   // S_schur starts as R_psi_psi
+    bool schur = false;
+    if (Nf > 0) {
+        fill(b.nrm.p, 2 * nb, 0.0, st);
+        sumsq_block(b.U.p, Nt, tt, 0, Np, Np, Nf, b.nrm.p, nb, st);
+        sumsq_block(b.U.p, Nt, tt, Np, 0, Nf, Np, b.nrm.p + nb, nb, st);
+        std::vector<double> nr(2 * nb);
+        SEHF_HIP_CHECK(hipMemcpyAsync(nr.data(), b.nrm.p, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        for (int e = 0; e < nb; ++e) if (!(std::sqrt(nr[e]) < 1e-14 && std::sqrt(nr[nb + e]) < 1e-14)) schur = true;
+    }
+    if (schur) {
+        get_block(b.Rff.p, b.U.p, Nt, tt, Np, Np, Nf, Nf, nb, st);
+        gj(b.Rff.p, b.Rffi.p, Nf, false);
+        Gemm g;   // T1 = R_psi_f R_ff^{-1}
+        g.M = Np; g.N = Nf; g.K = Nf;
+        g.A = b.U.p + static_cast<size_t>(Np) * Nt; g.lda = Nt; g.sA = tt;
+        g.B = b.Rffi.p; g.ldb = Nf; g.sB = static_cast<int64_t>(Nf) * Nf;
+        g.C = b.T1.p; g.ldc = Np; g.sC = pf; g.batch = nb;
+        gemm(g, st);
+        Gemm g2;  // S -= T1 R_f_psi
+        g2.M = Np; g2.N = Np; g2.K = Nf;
+        g2.A = b.T1.p; g2.lda = Np; g2.sA = pf;
+        g2.B = b.U.p + Np; g2.ldb = Nt; g2.sB = tt;
+        g2.C = b.Am.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.beta = 1.0; g2.batch = nb;
+        gemm(g2, st);
+    }
+    symmetrize(b.Am.p, Np, nb, st);
+    SEHF_HIP_CHECK(hipMemcpyAsync(b.T.p, b.Am.p, nb * pp * sizeof(double), hipMemcpyDeviceToDevice, st));
+    kme_mats(b.Am.p, b.Bm.p, b.T.p, b.tab.p, Np, nb, st);
+    gj(b.Bm.p, b.Psi.p, Np, false);          // Psi <- Bmat^{-1}
+    {
+        Gemm g;   // K = Bmat^{-1} Amat
+        g.M = Np; g.N = Np; g.K = Np;
+        g.A = b.Psi.p; g.lda = Np; g.sA = pp; g.B = b.Am.p; g.ldb = Np; g.sB = pp;
+        g.C = b.Kd.p; g.ldc = Np; g.sC = pp; g.batch = nb;
+        gemm(g, st);
+    }
+    symmetrize(b.Kd.p, Np, nb, st);
+    // ---------------- outer boundary (BackPropagator.cpp:128-197, :538-551) ----------------
+    psi_boundary(b.P.p, b.Kd.p, b.jy.p, Np, nb, st);
+    {
+        Gemm g;   // Vpsi = V_N psi_N
+        g.M = Np; g.N = Np; g.K = Np;
+        g.A = s->VN.p; g.lda = Np; g.sA = 0; g.B = b.P.p; g.ldb = Np; g.sB = pp;
+        g.C = b.Vpsi.p; g.ldc = Np; g.sC = pp; g.batch = nb;
+        gemm(g, st);
+    }
+    if (Nf > 0 && N < s->n_tr) {
+        Gemm g;   // G = B_N^T psi_N
+        g.M = Nf; g.N = Np; g.K = Np;
+        g.A = s->BN.p; g.lda = Np; g.sA = 0; g.tA = 1; g.B = b.P.p; g.ldb = Np; g.sB = pp;
+        g.C = b.G.p; g.ldc = Nf; g.sC = pf; g.batch = nb;
+        gemm(g, st);
+        row_scale(b.G.p, b.G.p, s->negDinvN.p, Nf, Np, nb, st);   // f_N = -D^{-1} B^T psi
+        Gemm g2;  // Bf = B_N f_N
+        g2.M = Np; g2.N = Np; g2.K = Nf;
+        g2.A = s->BN.p; g2.lda = Np; g2.sA = 0; g2.B = b.G.p; g2.ldb = Nf; g2.sB = pf;
+        g2.C = b.Bf.p; g2.ldc = Np; g2.sC = pp; g2.batch = nb;
+        gemm(g2, st);
+    } else {
+        fill(b.Bf.p, pp * nb, 0.0, st);
+    }
+    z_boundary(b.Z.p, b.P.p, b.Vpsi.p, b.Bf.p, b.cE.p, s->c6, Np, Nt, nb, st);
+    fill(b.acc.p, static_cast<size_t>(ns) * Np * nb, 0.0, st);
+    fill(b.YJ.p, pp * nb, 0.0, st);
+    fill(b.YY.p, pp * nb, 0.0, st);
+    auto dip_accum = [&](int n, const double* psi) {
+        Gemm g;   // acc += rows_n psi_n
+        g.M = ns; g.N = Np; g.K = Np;
+        g.A = s->Drows.p + static_cast<size_t>(n) * ns * Np; g.lda = ns; g.sA = 0;
+        g.B = psi; g.ldb = Np; g.sB = pp;
+        g.C = b.acc.p; g.ldc = ns; g.sC = static_cast<int64_t>(ns) * Np;
+        g.beta = 1.0; g.batch = nb;
+        gemm(g, st);
+    };
+    if (N < s->n_pts) dip_accum(N, b.P.p);
+    if (N >= n_fs && N <= n_fe) fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(N - n_fs) * pp, sWin, b.P.p,
+                                          b.Jc.p + static_cast<size_t>(N - n_fs) * Np, b.Yc.p + static_cast<size_t>(N - n_fs) * Np, sJc, Np, nb, st);
+    // ---------------- back-propagation by segments ----------------
+    for (int sg = n_seg - 1; sg >= 0; --sg) {
+        const int n0 = sg * L, n1 = std::min(n0 + L, N);
+        const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
+        for (int n = n0; n < n1; ++n) {
+            double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
+            double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
+            sinv_step(n, Sn);
+            if (n >= ns0) {
+                frp_step(n, Sn, prev, Rn);
+            } else if (n == 0) {
+                fill(Rn, tt * nb, 0.0, st);
+            } else {
+                diag_matrix(Rn, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+            }
+            prev = Rn;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
+            const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
+            Gemm g;   // Z <- Rinv_n Z
+            g.M = Nt; g.N = Np; g.K = Nt;
+            g.A = Rn; g.lda = Nt; g.sA = tt; g.B = b.Z.p; g.ldb = Nt; g.sB = static_cast<int64_t>(Nt) * Np;
+            g.C = b.Zt.p; g.ldc = Nt; g.sC = static_cast<int64_t>(Nt) * Np; g.batch = nb;
+            gemm(g, st);
+            std::swap(b.Z.p, b.Zt.p);
+            get_block(b.T.p, b.Z.p, Nt, static_cast<int64_t>(Nt) * Np, 0, 0, Np, Np, nb, st);
+            if (n < s->n_tr && Nf > 0) {
+                Gemm g2;  // rhs = Z_psi - (B D^{-1}) Z_f  (WInverseOperator.cpp:103-112)
+                g2.M = Np; g2.N = Np; g2.K = Nf;
+                g2.A = s->BD.p + static_cast<size_t>(n) * pf; g2.lda = Np; g2.sA = 0;
+                g2.B = b.Z.p + Np; g2.ldb = Nt; g2.sB = static_cast<int64_t>(Nt) * Np;
+                g2.C = b.T.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.beta = 1.0; g2.batch = nb;
+                gemm(g2, st);
+            }
+            Gemm g3;  // psi_n = Sinv_n rhs
+            g3.M = Np; g3.N = Np; g3.K = Np;
+            g3.A = Sn; g3.lda = Np; g3.sA = pp; g3.B = b.T.p; g3.ldb = Np; g3.sB = pp;
+            g3.C = b.Psi.p; g3.ldc = Np; g3.sC = pp; g3.batch = nb;
+            gemm(g3, st);
+            if (n < s->n_pts) dip_accum(n, b.Psi.p);
+            if (n >= n_fs && n <= n_fe)
+                fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(n - n_fs) * pp, sWin, b.Psi.p,
+                          b.Jc.p + static_cast<size_t>(n - n_fs) * Np, b.Yc.p + static_cast<size_t>(n - n_fs) * Np, sJc, Np, nb, st);
+        }
+    }
+    check_gj_info(b.gj, nb, st, "scan back-propagation");
+    // ---------------- asymptotic fit ----------------
+    fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
+    fill(b.red.p, 2 * nb, 0.0, st);
+    fit_resid(b.win.p, b.Am.p, b.Bm.p, b.Jc.p, b.Yc.p, b.coef.p, Np, n_fit, sWin, sJc, b.red.p, nb, st);
+    SEHF_HIP_CHECK(hipMemcpyAsync(b.T.p, b.Am.p, nb * pp * sizeof(double), hipMemcpyDeviceToDevice, st));
+    gj(b.T.p, b.Ainv.p, Np, false);
+    check_gj_info(b.gj, nb, st, "fitted A matrix");
+    {
+        Gemm g;   // K_fit = B A^{-1}
+        g.M = Np; g.N = Np; g.K = Np;
+        g.A = b.Bm.p; g.lda = Np; g.sA = pp; g.B = b.Ainv.p; g.ldb = Np; g.sB = pp;
+        g.C = b.Kfit.p; g.ldc = Np; g.sC = pp; g.batch = nb;
+        gemm(g, st);
+    }
+    // M = ((A - iB)^{-1})^dagger via the real 2Np system
+    w2_assemble(b.W2.p, b.Am.p, b.Bm.p, Np, nb, st);
+    gj(b.W2.p, b.W2i.p, 2 * Np, false);
+    const int64_t w2s = 4 * static_cast<int64_t>(pp), w2ld = 2 * Np;
+    auto mgemm = [&](double* C, const double* A, double alpha, const double* Bp, int64_t ldb, int ncol) {
+        Gemm g;   // C (Np x ncol) = alpha * A^T * op(B)   (op(B) = rows of acc, transposed)
+        g.M = Np; g.N = ncol; g.K = Np;
+        g.A = A; g.lda = w2ld; g.sA = w2s; g.tA = 1;
+        g.B = Bp; g.ldb = ldb; g.sB = static_cast<int64_t>(ns) * Np; g.tB = 1;
+        g.C = C; g.ldc = Np; g.sC = static_cast<int64_t>(Np) * ncol; g.alpha = alpha; g.batch = nb;
+        gemm(g, st);
+    };
+    mgemm(b.DRre.p, b.W2i.p, 1.0, b.acc.p, ns, 6);
+    mgemm(b.DRim.p, b.W2i.p + Np, -1.0, b.acc.p, ns, 6);
+    mgemm(b.ore.p, b.W2i.p, 1.0, b.acc.p + 6, ns, n_occ);
+    mgemm(b.oim.p, b.W2i.p + Np, -1.0, b.acc.p + 6, ns, n_occ);
+    dipole_final(b.Dre.p, b.Dim.p, b.DRre.p, b.DRim.p, b.ore.p, b.oim.p, s->dcorr.p, Np, n_occ, nb, st);
+    check_gj_info(b.gj, nb, st, "dipole normalisation");
+    // ---------------- download ----------------
+    auto dl = [&](double* dst, const double* src, size_t per) {
+        if (dst) SEHF_HIP_CHECK(hipMemcpyAsync(dst + static_cast<size_t>(out_off) * per, src, per * nb * sizeof(double),
+                                               hipMemcpyDeviceToHost, st));
+    };
+    dl(out->K, b.Kd.p, pp);
+    dl(out->A, b.Am.p, pp);
+    dl(out->B, b.Bm.p, pp);
+    dl(out->K_fit, b.Kfit.p, pp);
+    std::vector<double> acc(static_cast<size_t>(ns) * Np * nb), dre(6 * static_cast<size_t>(Np) * nb), dim(dre.size()),
+        drre(dre.size()), drim(dre.size()), red(2 * nb);
+    SEHF_HIP_CHECK(hipMemcpyAsync(acc.data(), b.acc.p, acc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(dre.data(), b.Dre.p, dre.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(dim.data(), b.Dim.p, dim.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(drre.data(), b.DRre.p, drre.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(drim.data(), b.DRim.p, drim.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(red.data(), b.red.p, red.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> Ah(pp * nb), Kh(pp * nb);
+    SEHF_HIP_CHECK(hipMemcpyAsync(Ah.data(), b.Am.p, Ah.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipMemcpyAsync(Kh.data(), b.Kfit.p, Kh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    for (int e = 0; e < nb; ++e) {
+        const int E_ = out_off + e;
+        const double* a = acc.data() + static_cast<size_t>(e) * ns * Np;
+        for (int slot = 0; slot < 6; ++slot)
+            for (int be = 0; be < Np; ++be) {
+                const size_t o = static_cast<size_t>(e) * 6 * Np + static_cast<size_t>(slot) * Np + be;
+                const size_t od = (static_cast<size_t>(E_) * 6 + slot) * Np + be;
+                if (out->d_raw) out->d_raw[od] = a[slot + static_cast<size_t>(be) * ns];
+                if (out->D_raw) { out->D_raw[2 * od] = drre[o]; out->D_raw[2 * od + 1] = drim[o]; }
+                if (out->D) { out->D[2 * od] = dre[o]; out->D[2 * od + 1] = dim[o]; }
+            }
+        if (out->b_overlap)
+            for (int al = 0; al < n_occ; ++al)
+                for (int be = 0; be < Np; ++be)
+                    out->b_overlap[(static_cast<size_t>(E_) * n_occ + al) * Np + be] = a[6 + al + static_cast<size_t>(be) * ns];
+        if (out->diag) {
+            double* d = out->diag + static_cast<size_t>(E_) * 16;
+            std::fill(d, d + 16, 0.0);
+            const double* A = Ah.data() + static_cast<size_t>(e) * pp;
+            const double* K = Kh.data() + static_cast<size_t>(e) * pp;
+            double ami = 0.0, ks_ = 0.0;
+            for (int j = 0; j < Np; ++j)
+                for (int i = 0; i < Np; ++i) {
+                    ami = std::max(ami, std::fabs(A[i + static_cast<size_t>(j) * Np] - (i == j ? 1.0 : 0.0)));
+                    ks_ = std::max(ks_, std::fabs(K[i + static_cast<size_t>(j) * Np] - K[j + static_cast<size_t>(i) * Np]));
+                }
+            d[0] = red[2 * e] / std::max(red[2 * e + 1], 1e-30);
+            d[1] = red[2 * e];
+            d[2] = ks_;
+            d[3] = ami;
+            d[4] = s->n_tr; d[5] = s->n_start; d[6] = n_fs; d[7] = n_fe;
+            d[8] = shA[e]; d[9] = shS[e]; d[10] = n_closed[e];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
+    return guarded([&] {
+        auto s = std::make_unique<sehf_scan>();
+        s->ctx = ctx;
+        s->st = sehf_ctx_stream(ctx);
+        s->S = std::make_unique<Setup>(*p);
+        const Setup& S = *s->S;
+        s->Np = S.Np; s->Nf = S.Nf; s->Nt = S.Nt; s->Nr = S.Nr; s->n_tr = S.n_tr; s->n_start = S.n_start;
+        s->h = S.dr; s->c6 = S.dr * S.dr / 6.0; s->nslot = 6 + S.n_occ;
+        build_shared(s.get());
+        *out = s.release();
+    });
+}
+
+int sehf_scan_info(const sehf_scan* s, int* info) {
+    return guarded([&] {
+        info[0] = s->Np; info[1] = s->Nf; info[2] = s->Nt; info[3] = s->n_tr;
+        info[4] = s->n_start; info[5] = s->n_pts; info[6] = s->Nr; info[7] = s->S->l_exch;
+    });
+}
+
+int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out) {
+    return guarded([&] {
+        if (nk <= 0) return;
+        for (int i = 0; i < nk; ++i)
+            if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_run: every k must be > 0");
+        const int N = s->Nr - 1;
+        int L = seg_len > 0 ? seg_len : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
+        L = std::max(L, s->n_start + 1);
+        const int n_seg = (N + L - 1) / L;
+        const int n_fit = 200;
+        int B = batch > 0 ? std::min(batch, nk) : nk;
+        alloc_batch(s, B, L, n_seg, n_fit);
+        if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
+        for (int e0 = 0; e0 < nk; e0 += B) {
+            const int nb = std::min(B, nk - e0);
+            run_batch(s, k + e0, nb, L, out, e0);
+        }
+    });
+}
+
+void sehf_scan_destroy(sehf_scan* s) { delete s; }
+
+}  // extern "C"

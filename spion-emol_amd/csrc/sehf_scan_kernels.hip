@@ -1,0 +1,365 @@
+// sehf_scan_kernels.hip -- elementwise kernels of the batched energy scan.
+// Every kernel takes `batch` energies at fixed strides; matrices are
+// column-major.  The O(N^3) work of the scan runs in k_dgemm and the
+// Gauss-Jordan kernels (sehf_kernels.hip); these kernels assemble operands
+// and reduce results around them.
+#include "sehf_scan_kernels.h"
+
+namespace sehf {
+
+// U = 12 W^{-1} - 10 I - R_prev, except the psi-f blocks (added by
+// add_psi_f_blocks / a GEMM):  psi-psi: 12 Sinv - 10 I - R;
+// f-f: diag(12 Dinv - 10) - R;  psi-f, f-psi: -R.
+__global__ void k_init_U(double* U, const double* R, const double* Sinv, const double* Dinv, int Np,
+                         int Nt, int64_t sU, int64_t sR, int64_t sS) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Nt) return;
+    const int64_t o = i + (int64_t)j * Nt;
+    double v = -R[b * sR + o];
+    if (i < Np && j < Np) v += 12.0 * Sinv[b * sS + i + (int64_t)j * Np] - (i == j ? 10.0 : 0.0);
+    else if (i >= Np && j >= Np && i == j) v += 12.0 * Dinv[i - Np] - 10.0;
+    U[b * sU + o] = v;
+}
+void init_U(double* U, const double* R, const double* Sinv, const double* Dinv, int Np, int Nt,
+            int batch, hipStream_t s) {
+    dim3 g((Nt + 255) / 256, Nt, batch);
+    hipLaunchKernelGGL(k_init_U, g, dim3(256), 0, s, U, R, Sinv, Dinv, Np, Nt, (int64_t)Nt * Nt,
+                       (int64_t)Nt * Nt, (int64_t)Np * Np);
+}
+
+// U_psi_f(i, j) += a X(i, j),  U_f_psi(j, i) += a X(i, j)   (X: Np x Nf)
+__global__ void k_add_psi_f(double* U, const double* X, double a, int Np, int Nf, int Nt, int64_t sU,
+                            int64_t sX) {
+    __shared__ double tile[32][33];
+    const int64_t b = blockIdx.z;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    double* Ub = U + b * sU;
+    const double* Xb = X + b * sX;
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int i = i0 + tx, j = j0 + yy;
+        if (i < Np && j < Nf) {
+            const double v = a * Xb[i + (int64_t)j * Np];
+            tile[yy][tx] = v;
+            Ub[i + (int64_t)(Np + j) * Nt] += v;
+        }
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int j = j0 + tx, i = i0 + yy;   // U(Np + j, i)
+        if (i < Np && j < Nf) Ub[(Np + j) + (int64_t)i * Nt] += tile[tx][yy];
+    }
+}
+void add_psi_f_blocks(double* U, const double* X, double a, int Np, int Nf, int Nt, int batch,
+                      hipStream_t s) {
+    dim3 g((Np + 31) / 32, (Nf + 31) / 32, batch);
+    hipLaunchKernelGGL(k_add_psi_f, g, dim3(256), 0, s, U, X, a, Np, Nf, Nt, (int64_t)Nt * Nt,
+                       (int64_t)Np * Nf);
+}
+
+// Per-energy diagonal shift of a shared matrix: Y_b = M0 + shift[b] I.
+// shift is read from device memory (shift + b*sstride).
+__global__ void k_shift_copy(double* Y, const double* M0, const double* shift, int n, int64_t sY) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    double v = M0[i + (int64_t)j * n];
+    if (i == j) v += shift[b];
+    Y[b * sY + i + (int64_t)j * n] = v;
+}
+void shift_copy(double* Y, const double* M0, const double* shift, int n, int batch, hipStream_t s) {
+    dim3 g((n + 255) / 256, n, batch);
+    hipLaunchKernelGGL(k_shift_copy, g, dim3(256), 0, s, Y, M0, shift, n, (int64_t)n * n);
+}
+
+// Schur matrix of energy b at one radial point (SchurInverter.cpp:254-372):
+// off-diagonal from the shared S0 = -(h^2/6)V - B D^{-1} B^T; the diagonal is
+// rebuilt in the reference's grouping
+//   S_ii = ((((1 + cE_b) - cV_i) + dA_b) - bdb_i) + dS_b
+// because the energy enters only through cE ~ 1e-7 on an O(1) diagonal and a
+// different grouping would shift the effective energy by ~1e-9.
+__global__ void k_sinv_matrix(double* Y, const double* S0, const double* cvd, const double* bdb,
+                              const double* cE, const double* dA, const double* dS, int n, int64_t sY) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    double v;
+    if (i == j) v = ((((1.0 + cE[b]) - cvd[i]) + dA[b]) - bdb[i]) + dS[b];
+    else v = S0[i + (int64_t)j * n];
+    Y[b * sY + i + (int64_t)j * n] = v;
+}
+void sinv_matrix(double* Y, const double* S0, const double* cvd, const double* bdb, const double* cE,
+                 const double* dA, const double* dS, int n, int batch, hipStream_t s) {
+    dim3 g((n + 255) / 256, n, batch);
+    hipLaunchKernelGGL(k_sinv_matrix, g, dim3(256), 0, s, Y, S0, cvd, bdb, cE, dA, dS, n, (int64_t)n * n);
+}
+
+// out[b*r + i] = sum_j X_b(i, j) Y_b(i, j)   (r x c blocks, leading dim r)
+__global__ void k_row_dot(double* out, const double* X, const double* Y, int r, int c) {
+    const int64_t b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r) return;
+    const int64_t o = b * (int64_t)r * c;
+    double s = 0.0;
+    for (int j = 0; j < c; ++j) s += X[o + i + (int64_t)j * r] * Y[o + i + (int64_t)j * r];
+    out[b * r + i] = s;
+}
+void row_dot(double* out, const double* X, const double* Y, int r, int c, int batch, hipStream_t s) {
+    dim3 g((r + 255) / 256, batch);
+    hipLaunchKernelGGL(k_row_dot, g, dim3(256), 0, s, out, X, Y, r, c);
+}
+
+// Batched analytic diagonal: R_b = diag(d_b) (n x n, d stride n).
+__global__ void k_diag_matrix(double* R, const double* d, int n, int64_t sR) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    R[b * sR + i + (int64_t)j * n] = (i == j) ? d[b * n + i] : 0.0;
+}
+void diag_matrix(double* R, const double* d, int n, int batch, hipStream_t s) {
+    dim3 g((n + 255) / 256, n, batch);
+    hipLaunchKernelGGL(k_diag_matrix, g, dim3(256), 0, s, R, d, n, (int64_t)n * n);
+}
+
+// Fit accumulation for window point i (AsymptoticAmplitudes.cpp:255-258):
+//   YJ += diag(Jc(i,:)) psi, YY += diag(Yc(i,:)) psi; the window copy of psi
+//   is kept for the residual pass.
+__global__ void k_fit_accum(double* YJ, double* YY, double* win, const double* psi, const double* Jc,
+                            const double* Yc, int Np, int64_t sM, int64_t sWin, int64_t sC) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (mu >= Np) return;
+    const int64_t o = mu + (int64_t)j * Np;
+    const double p = psi[b * sM + o];
+    YJ[b * sM + o] += Jc[b * sC + mu] * p;
+    YY[b * sM + o] += Yc[b * sC + mu] * p;
+    win[b * sWin + o] = p;
+}
+void fit_accum(double* YJ, double* YY, double* win, int64_t sWin, const double* psi, const double* Jc,
+               const double* Yc, int64_t sC, int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_fit_accum, g, dim3(256), 0, s, YJ, YY, win, psi, Jc, Yc, Np, (int64_t)Np * Np,
+                       sWin, sC);
+}
+
+// A = m00 YJ + m01 YY, B = m10 YJ + m11 YY per row mu; closed rows A = delta, B = 0.
+// coef layout per energy: [5][Np] = m00, m01, m10, m11, closed.
+__global__ void k_fit_solve(double* A, double* B, const double* YJ, const double* YY, const double* coef,
+                            int Np) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (mu >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = mu + (int64_t)j * Np;
+    const double* c = coef + b * 5 * Np;
+    if (c[4 * Np + mu] != 0.0) {
+        A[b * s + o] = (mu == j) ? 1.0 : 0.0;
+        B[b * s + o] = 0.0;
+        return;
+    }
+    const double yj = YJ[b * s + o], yy = YY[b * s + o];
+    A[b * s + o] = c[mu] * yj + c[Np + mu] * yy;
+    B[b * s + o] = c[2 * Np + mu] * yj + c[3 * Np + mu] * yy;
+}
+void fit_solve(double* A, double* B, const double* YJ, const double* YY, const double* coef, int Np,
+               int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_fit_solve, g, dim3(256), 0, s, A, B, YJ, YY, coef, Np);
+}
+
+__device__ inline void amax_pos(double* addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), __double_as_longlong(v));
+}
+
+// Residual pass (AsymptoticAmplitudes.cpp:284-305): out[b*2+0] = max |fit - psi|
+// over open rows, out[b*2+1] = max |psi| over the window.
+__global__ void k_fit_resid(const double* win, const double* A, const double* B, const double* Jc,
+                            const double* Yc, const double* coef, int Np, int nfit, int64_t sWin,
+                            int64_t sC, double* out) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    double r = 0.0, pm = 0.0;
+    if (mu < Np) {
+        const int64_t s = (int64_t)Np * Np, o = mu + (int64_t)j * Np;
+        const bool closed = coef[b * 5 * Np + 4 * Np + mu] != 0.0;
+        const double a = A[b * s + o], bb = B[b * s + o];
+        for (int i = 0; i < nfit; ++i) {
+            const double p = win[b * sWin + (int64_t)i * s + o];
+            pm = fmax(pm, fabs(p));
+            if (!closed) {
+                const double f = Jc[b * sC + (int64_t)i * Np + mu] * a + Yc[b * sC + (int64_t)i * Np + mu] * bb;
+                r = fmax(r, fabs(f - p));
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) { r = fmax(r, __shfl_xor(r, off)); pm = fmax(pm, __shfl_xor(pm, off)); }
+    if ((threadIdx.x & 63) == 0) {
+        if (r > 0.0) amax_pos(out + 2 * b, r);
+        if (pm > 0.0) amax_pos(out + 2 * b + 1, pm);
+    }
+}
+void fit_resid(const double* win, const double* A, const double* B, const double* Jc, const double* Yc,
+               const double* coef, int Np, int nfit, int64_t sWin, int64_t sC, double* out, int batch,
+               hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_fit_resid, g, dim3(256), 0, s, win, A, B, Jc, Yc, coef, Np, nfit, sWin, sC, out);
+}
+
+// In-place M <- 1/2 (M + M^T) (batched n x n).
+__global__ void k_symmetrize(double* M, int n, int64_t sM) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n || i <= j) return;
+    double* m = M + b * sM;
+    const double v = 0.5 * (m[i + (int64_t)j * n] + m[j + (int64_t)i * n]);
+    m[i + (int64_t)j * n] = v;
+    m[j + (int64_t)i * n] = v;
+}
+void symmetrize(double* M, int n, int batch, hipStream_t s) {
+    dim3 g((n + 255) / 256, n, batch);
+    hipLaunchKernelGGL(k_symmetrize, g, dim3(256), 0, s, M, n, (int64_t)n * n);
+}
+
+// Copy an r x c block out of a batched matrix with leading dim ld:
+// Y_b(i, j) = X_b(i0 + i, j0 + j).
+__global__ void k_get_block(double* Y, const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r,
+                            int c) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= r || j >= c) return;
+    Y[b * (int64_t)r * c + i + (int64_t)j * r] = X[b * sX + (i0 + i) + (int64_t)(j0 + j) * ldx];
+}
+void get_block(double* Y, const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r, int c,
+               int batch, hipStream_t s) {
+    dim3 g((r + 255) / 256, c, batch);
+    hipLaunchKernelGGL(k_get_block, g, dim3(256), 0, s, Y, X, ldx, sX, i0, j0, r, c);
+}
+
+// K-matrix matching matrices (KMatrixExtractor.cpp:146-157):
+//   Amat = W_out J_out - S (W_in J_in),  Bmat = S (W_in N_in) - W_out N_out
+// tab per energy: [6][Np] = J_in, J_out, Y_in, Y_out, W_in, W_out.
+__global__ void k_kme_mats(double* Am, double* Bm, const double* S, const double* tab, int Np) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = i + (int64_t)j * Np;
+    const double* t = tab + b * 6 * Np;
+    const double sv = S[b * s + o];
+    Am[b * s + o] = ((i == j) ? t[5 * Np + j] * t[Np + j] : 0.0) - sv * (t[4 * Np + j] * t[j]);
+    Bm[b * s + o] = sv * (t[4 * Np + j] * t[2 * Np + j]) - ((i == j) ? t[5 * Np + j] * t[3 * Np + j] : 0.0);
+}
+void kme_mats(double* Am, double* Bm, const double* S, const double* tab, int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_kme_mats, g, dim3(256), 0, s, Am, Bm, S, tab, Np);
+}
+
+// psi_N = J_N + N_N K  (KMatrixExtractor.cpp:200-224); jy per energy [2][Np].
+__global__ void k_psi_boundary(double* P, const double* K, const double* jy, int Np) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = i + (int64_t)j * Np;
+    const double* t = jy + b * 2 * Np;
+    P[b * s + o] = ((i == j) ? t[i] : 0.0) + t[Np + i] * K[b * s + o];
+}
+void psi_boundary(double* P, const double* K, const double* jy, int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_psi_boundary, g, dim3(256), 0, s, P, K, jy, Np);
+}
+
+// Y(i,j) = d[i] * X(i,j)  (batched r x c, shared d).
+__global__ void k_row_scale(double* Y, const double* X, const double* d, int r, int c) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= r) return;
+    const int64_t o = b * (int64_t)r * c + i + (int64_t)j * r;
+    Y[o] = d[i] * X[o];
+}
+void row_scale(double* Y, const double* X, const double* d, int r, int c, int batch, hipStream_t s) {
+    dim3 g((r + 255) / 256, c, batch);
+    hipLaunchKernelGGL(k_row_scale, g, dim3(256), 0, s, Y, X, d, r, c);
+}
+
+// Outer boundary (BackPropagator.cpp:128-197):
+//   Z_psi = psi + (h^2/6) E psi - (h^2/6) V psi + B f ;  Z_f = 0
+// Vpsi = V psi and Bf = B f precomputed; cE[b] = (h^2/6) E_b.
+__global__ void k_z_boundary(double* Z, const double* P, const double* Vpsi, const double* Bf,
+                             const double* cE, double c, int Np, int Nt) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Nt) return;
+    const int64_t s = (int64_t)Np * Np;
+    double v = 0.0;
+    if (i < Np) {
+        const int64_t o = i + (int64_t)j * Np;
+        const double p = P[b * s + o];
+        v = ((p + cE[b] * p) - c * Vpsi[b * s + o]) + Bf[b * s + o];
+    }
+    Z[b * (int64_t)Nt * Np + i + (int64_t)j * Nt] = v;
+}
+void z_boundary(double* Z, const double* P, const double* Vpsi, const double* Bf, const double* cE,
+                double c, int Np, int Nt, int batch, hipStream_t s) {
+    dim3 g((Nt + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_z_boundary, g, dim3(256), 0, s, Z, P, Vpsi, Bf, cE, c, Np, Nt);
+}
+
+// W2 = [[A, B], [-B, A]]  (2Np x 2Np), whose inverse's first block column is
+// [Re; Im] of (A - iB)^{-1}.
+__global__ void k_w2(double* W2, const double* A, const double* B, int Np) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int n2 = 2 * Np;
+    if (i >= n2) return;
+    const int64_t s = (int64_t)Np * Np;
+    const int ii = i % Np, jj = j % Np;
+    const double a = A[b * s + ii + (int64_t)jj * Np], bb = B[b * s + ii + (int64_t)jj * Np];
+    double v;
+    if (i < Np) v = (j < Np) ? a : bb;
+    else v = (j < Np) ? -bb : a;
+    W2[b * (int64_t)n2 * n2 + i + (int64_t)j * n2] = v;
+}
+void w2_assemble(double* W2, const double* A, const double* B, int Np, int batch, hipStream_t s) {
+    dim3 g((2 * Np + 255) / 256, 2 * Np, batch);
+    hipLaunchKernelGGL(k_w2, g, dim3(256), 0, s, W2, A, B, Np);
+}
+
+// D(mu, s) = D_raw(mu, s) - 2 sum_a o(mu, a) dcorr(s, a)   (complex, split re/im)
+// (DipoleMatrixElement.cpp:264-282 with spin factor 2).
+__global__ void k_dipole_final(double* Dre, double* Dim, const double* DRre, const double* DRim,
+                               const double* ore, const double* oim, const double* dcorr, int Np, int n_occ) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (mu >= Np) return;
+    const int64_t so = (int64_t)Np * 6, oo = (int64_t)Np * n_occ;
+    double sr = 0.0, si = 0.0;
+    for (int a = 0; a < n_occ; ++a) {
+        const double dc = dcorr[s * n_occ + a];
+        sr += 2.0 * ore[b * oo + mu + (int64_t)a * Np] * dc;
+        si += 2.0 * oim[b * oo + mu + (int64_t)a * Np] * dc;
+    }
+    const int64_t o = mu + (int64_t)s * Np;
+    Dre[b * so + o] = DRre[b * so + o] - sr;
+    Dim[b * so + o] = DRim[b * so + o] - si;
+}
+void dipole_final(double* Dre, double* Dim, const double* DRre, const double* DRim, const double* ore,
+                  const double* oim, const double* dcorr, int Np, int n_occ, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, 6, batch);
+    hipLaunchKernelGGL(k_dipole_final, g, dim3(256), 0, s, Dre, Dim, DRre, DRim, ore, oim, dcorr, Np, n_occ);
+}
+
+// sum of squares of a batched r x c block (for the ||R_psi_f|| test).
+__global__ void k_sumsq(const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r, int c, double* out) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    double v = 0.0;
+    if (i < r && j < c) { const double x = X[b * sX + (i0 + i) + (int64_t)(j0 + j) * ldx]; v = x * x; }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(out + b, v);
+}
+void sumsq_block(const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r, int c, double* out,
+                 int batch, hipStream_t s) {
+    dim3 g((r + 255) / 256, c, batch);
+    hipLaunchKernelGGL(k_sumsq, g, dim3(256), 0, s, X, ldx, sX, i0, j0, r, c, out);
+}
+
+}  // namespace sehf

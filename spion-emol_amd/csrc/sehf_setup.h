@@ -26,7 +26,7 @@ class Setup {
 public:
     explicit Setup(const sehf_problem& p);
 
-    const sehf_problem& P;
+    const sehf_problem P;   // copied; its arrays are read only during construction
     int l_cont, l_exch, l_orb, Np, n_sigma, n_lambda, l_exp, n_exp, n_occ, Nr, Nf, Nt;
     int n_tr = 0, n_start = 1;
     double dr;
@@ -34,6 +34,8 @@ public:
 
     // ---- per radial index (solver radius r = (n+1) dr; potential radius n dr)
     void potential(int ir, double* V) const;   // Np x Np, col-major, r = ir*dr
+    // V(n) and B(n) for n in [n0, n0+count), in parallel over n.
+    void build_chunk(int n0, int count, double* V, double* B) const;
     void exchange_B(int ir, double* B) const;  // Np x Nf, col-major, (h^2/12) Q
     void dinv(int n, double* d) const;         // Nf: 1/max(1 - h^2/12 l(l+1)/r^2, W_min)
     double polarization(double r) const;

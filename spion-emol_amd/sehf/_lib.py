@@ -40,6 +40,18 @@ SIGNATURES = [
     ("sehf_dme_create", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("sehf_dme_step", ctypes.c_int, [_vp, _dp, _dp, _dp]),
     ("sehf_dme_destroy", None, [_vp]),
+    ("sehf_scan_create", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    ("sehf_scan_info", ctypes.c_int, [_vp, _ip]),
+    ("sehf_scan_run", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    ("sehf_scan_destroy", None, [_vp]),
+    ("sehf_host_potential", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    ("sehf_host_exchange_B", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    ("sehf_host_sizes", ctypes.c_int, [_vp, _ip]),
+    ("sehf_host_dipole_tables", ctypes.c_int, [_vp, _dp, _dp, _ip]),
+    ("sehf_host_ric_j", ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
+    ("sehf_host_ric_y", ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
+    ("sehf_host_real_gaunt", ctypes.c_double, [ctypes.c_int] * 6),
+    ("sehf_host_min_eig", ctypes.c_double, [ctypes.c_int, _dp]),
 ]
 
 _lib = None

@@ -1,0 +1,111 @@
+"""Batched energy scan through the C ABI (sehf_scan_*).
+
+`solve_scan(problem, ks)` runs the whole hot path -- Schur inverse, forward
+ratio recursion, K matrix, back-propagation, asymptotic fit and the six
+dipole slices -- for every k on the device and returns one dict per energy
+with the DipoleEnergyPayload fields (src/scatt/DipoleIO.hpp:82-112).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import check, load
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Problem_C(ctypes.Structure):
+    _fields_ = [("l_cont", ctypes.c_int), ("n_occ", ctypes.c_int), ("Nr", ctypes.c_int),
+                ("dr", ctypes.c_double), ("n_atoms", ctypes.c_int), ("Z", _ip), ("xyz", _dp),
+                ("nlm_vh", ctypes.c_int), ("V_H", _dp), ("n_orb", ctypes.c_int),
+                ("nlm_orb", ctypes.c_int), ("psi_lm", _dp), ("has_init", ctypes.c_int),
+                ("init_psi_lm", _dp), ("pol_kind", ctypes.c_int), ("pol_alpha", ctypes.c_double),
+                ("pol_rc", ctypes.c_double), ("allow_closed", ctypes.c_int)]
+
+
+class ScanOut_C(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D_raw", "D",
+                                   "d_corr", "diag")]
+
+
+def _register(L):
+    if getattr(L, "_scan_registered", False):
+        return L
+    L.sehf_scan_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(Problem_C), ctypes.POINTER(ctypes.c_void_p)]
+    L.sehf_scan_create.restype = ctypes.c_int
+    L.sehf_scan_info.argtypes = [ctypes.c_void_p, _ip]
+    L.sehf_scan_info.restype = ctypes.c_int
+    L.sehf_scan_run.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ScanOut_C)]
+    L.sehf_scan_run.restype = ctypes.c_int
+    L.sehf_scan_destroy.argtypes = [ctypes.c_void_p]
+    L.sehf_scan_destroy.restype = None
+    L._scan_registered = True
+    return L
+
+
+def _c_problem(p, keep):
+    def arr(x, dt=np.float64):
+        y = np.ascontiguousarray(x, dtype=dt)
+        keep.append(y)
+        return y
+    Z = arr(p.Z, np.int32)
+    xyz = arr(p.xyz)
+    vh = arr(p.V_H)
+    psi = arr(p.psi_lm)
+    init = arr(p.init_psi_lm) if p.init_psi_lm is not None else None
+    dp = lambda a: a.ctypes.data_as(_dp) if a is not None else None  # noqa: E731
+    return Problem_C(p.l_cont, p.n_occ, p.Nr, p.dr, len(Z), Z.ctypes.data_as(_ip), dp(xyz), vh.shape[0],
+                     dp(vh), psi.shape[0], psi.shape[1], dp(psi), 1 if init is not None else 0, dp(init),
+                     p.pol_kind, p.pol_alpha, p.pol_rc, 1 if p.allow_closed else 0)
+
+
+class Scan:
+    """Energy-independent device state for one molecule (sehf_scan handle)."""
+
+    def __init__(self, problem, ctx=None):
+        from .steppers import GpuContext
+        L = _register(load())
+        self.ctx = ctx or GpuContext(True)
+        self.p = problem
+        keep = []
+        cp = _c_problem(problem, keep)
+        h = ctypes.c_void_p()
+        check(L.sehf_scan_create(self.ctx._h, ctypes.byref(cp), ctypes.byref(h)), "sehf_scan_create")
+        self._h = h
+        info = (ctypes.c_int * 8)()
+        check(L.sehf_scan_info(h, info))
+        self.info = dict(zip(("n_psi", "n_f", "n_tot", "n_transition", "n_start", "n_pts", "Nr", "l_exch"), list(info)))
+
+    def run(self, ks, batch: int = 0, seg_len: int = 0):
+        L = load()
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        nk, N, no = len(ks), self.p.n_psi, self.p.n_occ
+        o = {"K": np.zeros((nk, N, N)), "A": np.zeros((nk, N, N)), "B": np.zeros((nk, N, N)),
+             "K_fit": np.zeros((nk, N, N)), "b_overlap": np.zeros((nk, no, N)), "d_raw": np.zeros((nk, 6, N)),
+             "D_raw": np.zeros((nk, 6, N), dtype=np.complex128), "D": np.zeros((nk, 6, N), dtype=np.complex128),
+             "d_corr": np.zeros((6, no)), "diag": np.zeros((nk, 16))}
+        out = ScanOut_C(*[o[n].ctypes.data_as(_dp) for n in ("K", "A", "B", "K_fit", "b_overlap", "d_raw",
+                                                             "D_raw", "D", "d_corr", "diag")])
+        check(L.sehf_scan_run(self._h, ks.ctypes.data_as(_dp), nk, batch, seg_len, ctypes.byref(out)),
+              "sehf_scan_run")
+        res = []
+        for e in range(nk):
+            # matrices come back column-major per energy
+            res.append({"k": float(ks[e]), "K": o["K"][e].T.copy(), "A": o["A"][e].T.copy(),
+                        "B": o["B"][e].T.copy(), "K_fit": o["K_fit"][e].T.copy(),
+                        "b_overlap": o["b_overlap"][e].T.copy(), "d_raw": o["d_raw"][e], "D_raw": o["D_raw"][e],
+                        "D": o["D"][e], "d_corr": o["d_corr"], "diag": o["diag"][e]})
+        return res
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().sehf_scan_destroy(self._h)
+            self._h = None
+
+
+def solve_scan(problem, ks, batch: int = 0, seg_len: int = 0, ctx=None):
+    return Scan(problem, ctx).run(ks, batch=batch, seg_len=seg_len)
