@@ -142,7 +142,17 @@ int  sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out);
 /* info[8] = N_psi, N_f, N_total, n_transition, n_start, dipole points, Nr, l_exch */
 int  sehf_scan_info(const sehf_scan* s, int* info);
 int  sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out);
+/* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
+ * fit + dipole, per-batch host tables. */
+int  sehf_scan_stats(const sehf_scan* s, double* t);
 void sehf_scan_destroy(sehf_scan* s);
+
+/* Kernel timing: between begin and end, every `every`-th FP64 GEMM and
+ * Gauss-Jordan panel launch is bracketed by HIP events on its own stream.
+ * out[6] = {gemm ms, gemm flop, gemm launches, panel ms, panel flop,
+ * panel launches} over the sampled launches. */
+int  sehf_prof_begin(int every);
+int  sehf_prof_end(double* out);
 
 /* ---- host-only inputs (no device needed) -------------------------------
  * The energy-independent matrices and scalar tables the scan uploads, for

@@ -1,7 +1,10 @@
 // sehf_host_api.cpp -- host-only entry points (no device needed): the
 // energy-independent inputs and scalar tables the scan uploads, exposed so
 // they can be checked on a machine without a GPU.
+#include <algorithm>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,7 +56,15 @@ int sehf_host_dipole_tables(const sehf_problem* p, double* rows, double* dcorr, 
 }
 double sehf_host_ric_j(int l, double x) { return ric_j(l, x); }
 double sehf_host_ric_y(int l, double x) { return ric_y(l, x); }
-double sehf_host_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3) { return real_gaunt(l1, m1, l2, m2, l3, m3); }
+double sehf_host_real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3) {
+    // the quadrature table the setup builds its coupling lists from
+    static std::mutex mu;
+    static std::unique_ptr<GauntTable> table;
+    const int need = std::max({l1, l2, l3, 0});
+    std::lock_guard<std::mutex> lock(mu);
+    if (!table || table->lmax() < need) table.reset(new GauntTable(std::max(need, 64)));
+    return (*table)(l1, m1, l2, m2, l3, m3);
+}
 double sehf_host_min_eig(int n, const double* a) { return sym_min_eig(n, a); }
 
 }  // extern "C"

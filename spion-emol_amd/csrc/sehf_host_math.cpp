@@ -113,6 +113,89 @@ double real_gaunt(int l1, int m1, int l2, int m2, int l3, int m3) {
     return pre * acc.real();
 }
 
+GauntTable::GauntTable(int lmax) : lmax_(lmax) {
+    nq_ = (3 * lmax) / 2 + 2;
+    // Gauss-Legendre nodes by Newton iteration on P_n
+    std::vector<double> x(nq_);
+    w_.assign(nq_, 0.0);
+    const int n = nq_;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) { const double p3 = p2; p2 = p1; p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j; }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            const double z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) < 1e-16) {
+                double q1 = 1.0, q2 = 0.0;
+                for (int j = 1; j <= n; ++j) { const double q3 = q2; q2 = q1; q1 = ((2.0 * j - 1.0) * z * q2 - (j - 1.0) * q3) / j; }
+                pp = n * (z * q1 - q2) / (z * z - 1.0);
+                break;
+            }
+        }
+        x[i] = -z; x[n - 1 - i] = z;
+        w_[i] = w_[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+    }
+    // normalised associated Legendre without Condon-Shortley phase (the theta
+    // part of Y^R, including the 1/sqrt(4 pi) of the m = 0 harmonic)
+    S_.assign(static_cast<size_t>((lmax + 1) * (lmax + 2) / 2) * nq_, 0.0);
+    for (int q = 0; q < nq_; ++q) {
+        const double ct = x[q], st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+        double pmm = 1.0 / std::sqrt(4.0 * M_PI);
+        for (int m = 0; m <= lmax; ++m) {
+            if (m > 0) pmm *= std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
+            double p0 = 0.0, p1 = pmm;
+            S_[(static_cast<size_t>(m * (m + 1) / 2 + m)) * nq_ + q] = pmm;
+            for (int l = m + 1; l <= lmax; ++l) {
+                double v;
+                if (l == m + 1) v = std::sqrt(2.0 * m + 3.0) * ct * pmm;
+                else {
+                    const double a = std::sqrt((4.0 * l * l - 1.0) / (static_cast<double>(l) * l - m * m));
+                    const double b = std::sqrt(((l - 1.0) * (l - 1.0) - m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                    v = a * (ct * p1 - b * p0);
+                }
+                p0 = p1; p1 = v;
+                S_[(static_cast<size_t>(l * (l + 1) / 2 + m)) * nq_ + q] = v;
+            }
+        }
+    }
+}
+
+double GauntTable::operator()(int l1, int m1, int l2, int m2, int l3, int m3) const {
+    if (l3 < std::abs(l1 - l2) || l3 > l1 + l2 || ((l1 + l2 + l3) & 1)) return 0.0;
+    if (std::abs(m1) > l1 || std::abs(m2) > l2 || std::abs(m3) > l3) return 0.0;
+    // phi integral of f1 f2 f3, f_0 = 1, f_{m>0} = sqrt2 cos(m phi),
+    // f_{m<0} = sqrt2 sin(|m| phi), via exponentials: sum over sign choices
+    // with s1 a + s2 b + s3 c = 0 of the product of coefficients, times 2 pi.
+    const int ms[3] = {m1, m2, m3};
+    std::complex<double> acc(0.0, 0.0);
+    for (int s1 = -1; s1 <= 1; s1 += 2)
+        for (int s2 = -1; s2 <= 1; s2 += 2)
+            for (int s3 = -1; s3 <= 1; s3 += 2) {
+                const int sg[3] = {s1, s2, s3};
+                int tot = 0;
+                std::complex<double> c(1.0, 0.0);
+                bool skip = false;
+                for (int t = 0; t < 3; ++t) {
+                    const int m = ms[t], a = std::abs(m);
+                    if (m == 0) { if (sg[t] < 0) { skip = true; break; } continue; }
+                    tot += sg[t] * a;
+                    c *= (m > 0) ? std::complex<double>(0.5 * std::sqrt(2.0), 0.0)
+                                 : std::complex<double>(0.0, -0.5 * std::sqrt(2.0) * sg[t]);
+                }
+                if (!skip && tot == 0) acc += c;
+            }
+    const double phi = 2.0 * M_PI * acc.real();
+    if (phi == 0.0) return 0.0;
+    const double* a = S_.data() + static_cast<size_t>(l1 * (l1 + 1) / 2 + std::abs(m1)) * nq_;
+    const double* b = S_.data() + static_cast<size_t>(l2 * (l2 + 1) / 2 + std::abs(m2)) * nq_;
+    const double* c = S_.data() + static_cast<size_t>(l3 * (l3 + 1) / 2 + std::abs(m3)) * nq_;
+    double th = 0.0;
+    for (int q = 0; q < nq_; ++q) th += w_[q] * a[q] * b[q] * c[q];
+    return th * phi;
+}
+
 double real_ylm(int l, int m, double theta, double phi) {
     if (l < 0 || std::abs(m) > l) return 0.0;
     const int am = std::abs(m);

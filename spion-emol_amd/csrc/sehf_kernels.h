@@ -69,4 +69,12 @@ void fill_int(int* p, size_t n, int v, hipStream_t s);
 // max_{b,i,j} |X[b](i,j) - X[b](j,i)| into *out (device double), n x n blocks.
 void max_asym(const double* X, int64_t ld, int64_t sX, int n, int batch, double* out, hipStream_t s);
 
+// Per-launch timing of k_dgemm (class 0) and the Gauss-Jordan panel kernel
+// (class 1) with HIP events on the launch stream, for every `every`-th launch
+// of each class.  prof_end waits for the recorded events and writes out[6] =
+// {gemm ms, gemm flop, gemm launches, panel ms, panel flop, panel launches}
+// over the sampled launches.
+void prof_begin(int every);
+void prof_end(double* out);
+
 }  // namespace sehf

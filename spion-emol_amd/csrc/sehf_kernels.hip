@@ -17,6 +17,7 @@
 #include "sehf_kernels.h"
 
 #include <climits>
+#include <vector>
 
 namespace sehf {
 
@@ -125,11 +126,66 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
             }
 }
 
+// ---------------------------------------------------------------------------
+// Optional per-launch timing (HIP events on the launch stream).
+// ---------------------------------------------------------------------------
+namespace {
+struct ProfRec { hipEvent_t a, b; double flops; int cls; };
+bool g_prof = false;
+int g_every = 1;
+long long g_count[2] = {0, 0};
+std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+hipEvent_t take_event() {
+    if (!g_pool.empty()) { hipEvent_t e = g_pool.back(); g_pool.pop_back(); return e; }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+}  // namespace
+
+void prof_begin(int every) {
+    g_prof = true;
+    g_every = every > 0 ? every : 1;
+    g_count[0] = g_count[1] = 0;
+    g_recs.clear();
+}
+
+void prof_end(double* out) {
+    g_prof = false;
+    for (int i = 0; i < 6; ++i) out[i] = 0.0;
+    for (auto& r : g_recs) {
+        (void)hipEventSynchronize(r.b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        out[3 * r.cls + 0] += ms;
+        out[3 * r.cls + 1] += r.flops;
+        out[3 * r.cls + 2] += 1.0;
+        g_pool.push_back(r.a);
+        g_pool.push_back(r.b);
+    }
+    g_recs.clear();
+}
+
+struct ProfScope {
+    bool on;
+    ProfRec r;
+    hipStream_t s;
+    ProfScope(hipStream_t st, double flops, int cls) : on(g_prof), s(st) {
+        if (on) on = (g_count[cls]++ % g_every) == 0;
+        if (on) { r.a = take_event(); r.b = take_event(); r.flops = flops; r.cls = cls; (void)hipEventRecord(r.a, s); }
+    }
+    ~ProfScope() {
+        if (on) { (void)hipEventRecord(r.b, s); g_recs.push_back(r); }
+    }
+};
+
 void gemm(const Gemm& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
     GemmK p{g.M, g.N, g.K, g.A, g.lda, g.sA, g.tA, g.B, g.ldb, g.sB, g.tB, g.C, g.ldc, g.sC,
             g.alpha, g.beta, g.zc0, g.zc1};
     dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);
+    ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, 0);
     hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, p);
 }
 
@@ -320,6 +376,7 @@ size_t gj_tbuf_elems(int n) { return static_cast<size_t>(n) * n; }
 
 static void gj_panel(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w,
                      const GjWork& W, hipStream_t s) {
+    ProfScope ps(s, 2.0 * n * static_cast<double>(w) * w * batch, 1);
     if (n <= 256) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(256), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
     else if (n <= 512) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(512), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
     else if (n <= 1024) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(1024), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);

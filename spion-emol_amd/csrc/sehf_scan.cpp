@@ -14,6 +14,7 @@
 // fit sums (AsymptoticAmplitudes.cpp:64-359) into the sweep, so psi is never
 // stored except for the ~200-point fit window.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -58,6 +59,7 @@ struct sehf_scan {
     std::vector<std::vector<double>> A0h, S0h;
     Batch bt;
     size_t dev_bytes_shared = 0;
+    double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
 };
 
 // context stream accessor (sehf_ctx is defined in sehf_capi.cpp)
@@ -255,7 +257,12 @@ void up(double* d, const std::vector<double>& h, hipStream_t st) {
     if (!h.empty()) SEHF_HIP_CHECK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
 }
 
+double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out, int out_off) {
+    const double tw0 = wall();
     Setup& S = *s->S;
     hipStream_t st = s->st;
     Batch& b = s->bt;
@@ -398,6 +405,8 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
         }
         gj(b.U.p, Rout, Nt, true);
     };
+    SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    const double tw1 = wall();
     // ---------------- forward sweep ----------------
     if (ns0 >= 2) diag_matrix(b.Rp.p, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
     else fill(b.Rp.p, tt * nb, 0.0, st);
@@ -413,6 +422,7 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
         }
     }
     check_gj_info(b.gj, nb, st, "scan forward sweep");
+    const double tw2 = wall();
     // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
     SEHF_HIP_CHECK(hipMemcpyAsync(b.Rc.p, b.Rp.p, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
     gj(b.Rc.p, b.U.p, Nt, false);   // U <- R = Rinv_{N-1}^{-1}
@@ -495,6 +505,8 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
     if (N < s->n_pts) dip_accum(N, b.P.p);
     if (N >= n_fs && N <= n_fe) fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(N - n_fs) * pp, sWin, b.P.p,
                                           b.Jc.p + static_cast<size_t>(N - n_fs) * Np, b.Yc.p + static_cast<size_t>(N - n_fs) * Np, sJc, Np, nb, st);
+    SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    const double tw3 = wall();
     // ---------------- back-propagation by segments ----------------
     for (int sg = n_seg - 1; sg >= 0; --sg) {
         const int n0 = sg * L, n1 = std::min(n0 + L, N);
@@ -542,6 +554,7 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
         }
     }
     check_gj_info(b.gj, nb, st, "scan back-propagation");
+    const double tw4 = wall();
     // ---------------- asymptotic fit ----------------
     fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
     fill(b.red.p, 2 * nb, 0.0, st);
@@ -595,6 +608,9 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
     SEHF_HIP_CHECK(hipMemcpyAsync(Ah.data(), b.Am.p, Ah.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     SEHF_HIP_CHECK(hipMemcpyAsync(Kh.data(), b.Kfit.p, Kh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    const double tw5 = wall();
+    s->t_stage[1] += tw2 - tw1; s->t_stage[2] += tw3 - tw2; s->t_stage[3] += tw4 - tw3;
+    s->t_stage[4] += tw5 - tw4; s->t_stage[5] += tw1 - tw0;
     for (int e = 0; e < nb; ++e) {
         const int E_ = out_off + e;
         const double* a = acc.data() + static_cast<size_t>(e) * ns * Np;
@@ -644,7 +660,9 @@ int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
         const Setup& S = *s->S;
         s->Np = S.Np; s->Nf = S.Nf; s->Nt = S.Nt; s->Nr = S.Nr; s->n_tr = S.n_tr; s->n_start = S.n_start;
         s->h = S.dr; s->c6 = S.dr * S.dr / 6.0; s->nslot = 6 + S.n_occ;
+        const double t0 = wall();
         build_shared(s.get());
+        s->t_stage[0] = wall() - t0;
         *out = s.release();
     });
 }
@@ -674,6 +692,18 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
             run_batch(s, k + e0, nb, L, out, e0);
         }
     });
+}
+
+int sehf_scan_stats(const sehf_scan* s, double* t) {
+    return guarded([&] { for (int i = 0; i < 8; ++i) t[i] = s->t_stage[i]; });
+}
+
+int sehf_prof_begin(int every) {
+    return guarded([&] { prof_begin(every); });
+}
+
+int sehf_prof_end(double* out) {
+    return guarded([&] { prof_end(out); });
 }
 
 void sehf_scan_destroy(sehf_scan* s) { delete s; }

@@ -82,6 +82,7 @@ void Setup::build_potential_gaunt() {
             for (int b = a; b < Np; ++b) { pair_i_[q] = a; pair_j_[q] = b; ++q; }
     }
     std::vector<std::vector<std::pair<int, double>>> rows(npairs);
+    const GauntTable gt(l_exp);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t q = 0; q < npairs; ++q) {
         const int a = pair_i_[q], b = pair_j_[q];
@@ -95,7 +96,7 @@ void Setup::build_potential_gaunt() {
                 if (am > ls) continue;
                 for (int sg = (am == 0 ? 1 : -1); sg <= 1; sg += 2) {
                     const int ms = sg * am;
-                    const double G = real_gaunt(l2, m2, ls, ms, l1, m1);
+                    const double G = gt(l2, m2, ls, ms, l1, m1);
                     if (std::fabs(G) > 1e-15) row.push_back({lm_index(ls, ms), G});
                 }
             }
@@ -117,6 +118,7 @@ void Setup::build_potential_gaunt() {
 void Setup::build_exchange_gaunt() {
     const int nrow = Np * n_sigma;
     std::vector<std::vector<std::pair<int, double>>> rows(nrow);
+    const GauntTable gt(l_orb);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int rr = 0; rr < nrow; ++rr) {
         const int mu = rr / n_sigma, s = rr % n_sigma;
@@ -131,7 +133,7 @@ void Setup::build_exchange_gaunt() {
                 if (am > ll) continue;
                 for (int sg = (am == 0 ? 1 : -1); sg <= 1; sg += 2) {
                     const int ml = sg * am;
-                    const double G = real_gaunt(lm, mm, ll, ml, ls, ms);
+                    const double G = gt(lm, mm, ll, ml, ls, ms);
                     if (std::fabs(G) > 1e-14) row.push_back({lm_index(ll, ml), G});
                 }
             }

@@ -44,6 +44,9 @@ SIGNATURES = [
     ("sehf_scan_info", ctypes.c_int, [_vp, _ip]),
     ("sehf_scan_run", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     ("sehf_scan_destroy", None, [_vp]),
+    ("sehf_scan_stats", ctypes.c_int, [_vp, _dp]),
+    ("sehf_prof_begin", ctypes.c_int, [ctypes.c_int]),
+    ("sehf_prof_end", ctypes.c_int, [_dp]),
     ("sehf_host_potential", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("sehf_host_exchange_B", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("sehf_host_sizes", ctypes.c_int, [_vp, _ip]),

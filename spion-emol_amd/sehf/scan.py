@@ -101,6 +101,11 @@ class Scan:
                         "D": o["D"][e], "d_corr": o["d_corr"], "diag": o["diag"][e]})
         return res
 
+    def stats(self) -> dict:
+        t = (ctypes.c_double * 8)()
+        check(load().sehf_scan_stats(self._h, t))
+        return dict(zip(("setup", "forward", "kmatrix", "backprop", "fit_dipole", "host_tables"), list(t)[:6]))
+
     def __del__(self):
         if getattr(self, "_h", None):
             load().sehf_scan_destroy(self._h)

@@ -85,6 +85,31 @@ def test_gaunt_vs_quadrature(O, H):
     assert worst_o < 1e-13 and worst_p < 1e-13
 
 
+def test_gaunt_high_l_product_vs_oracle(O, H):
+    """The product's Gauss-Legendre Gaunt table against the oracle's Racah
+    sums (pinned above against sympy) at the l of the l_cont = 30 scan:
+    l_mu, l_mu' <= 30, l_sigma <= 60, m triplets that pass the real-harmonic
+    selection rule |m_s| in {|m1|+|m2|, ||m1|-|m2||}."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    n_nz = 0
+    for _ in range(3000):
+        l1, l2 = (int(x) for x in rng.integers(0, 31, size=2))
+        m1 = int(rng.integers(-l1, l1 + 1))
+        m2 = int(rng.integers(-l2, l2 + 1))
+        l3 = int(rng.integers(abs(l1 - l2), l1 + l2 + 1))
+        am = abs(m1) + abs(m2) if rng.random() < 0.5 else abs(abs(m1) - abs(m2))
+        if am > l3:
+            continue
+        m3 = am if rng.random() < 0.5 else -am
+        o = O.lib().sehf_oracle_gaunt_real(l1, m1, l3, m3, l2, m2)
+        p = H.sehf_host_real_gaunt(l1, m1, l3, m3, l2, m2)
+        worst = max(worst, abs(o - p))
+        n_nz += abs(o) > 1e-15
+    assert n_nz > 500
+    assert worst < 2e-14, worst
+
+
 @pytest.mark.parametrize("which", ["oracle", "product"])
 def test_riccati_bessel(O, H, which):
     L = O.lib()
