@@ -44,6 +44,8 @@ struct GjWork {
 void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
                 bool symmetrize, const GjWork& w, hipStream_t s);
 size_t gj_tbuf_elems(int n);   // doubles per matrix needed in tbuf
+// Threads per Gauss-Jordan leaf workgroup (256, 512 or 1024; 0 = by order).
+void gj_set_leaf_threads(int t);
 
 // Y[b] = a * X[b] + c * Z[b] + d * I   (n x m column-major blocks; Z may be null).
 void axpbyi(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX, double a,

@@ -17,6 +17,8 @@
 #include "sehf_kernels.h"
 
 #include <climits>
+#include <cstdlib>
+#include <stdexcept>
 #include <vector>
 
 namespace sehf {
@@ -191,21 +193,43 @@ void gemm(const Gemm& g, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Gauss-Jordan leaf: pivots c .. c+w-1 (w <= 16) on columns [c, c+w) of the
-// row-major view.  Thread t owns rows t, t+T, ... (R rows).
+// row-major view, one workgroup per matrix, thread t owning rows t, t+T, ...
+// (R rows) of the panel in registers.
+//
+// Per pivot column: a max-|.| search (wave shuffles, then one LDS exchange
+// between waves), one LDS broadcast of the pivot row and of row k, and the
+// rank-1 elimination in registers -- two barriers and no global memory
+// traffic.  The row interchanges of the columns outside the panel are
+// applied once at the end as one composite permutation of the <= 2w rows
+// involved (loads of all source rows, then stores).
+//
+// Optional fused sibling updates (the recursion's block steps for a node of
+// two leaves):
+//   pre  (pre_w > 0): before factoring, apply pivot block Kp = [pre_c,
+//        pre_c+pre_w) to this panel:  X(O,J) += X(O,Kp) T,  X(Kp,J) =
+//        X(Kp,Kp) T,  T = X(Kp,J)_old.
+//   post (post_w > 0): after factoring and the row interchanges, apply this
+//        pivot block K = [c, c+w) to columns Jq = [post_c, post_c+post_w).
 // ---------------------------------------------------------------------------
 constexpr int GJW = 16;
 
-template <int R>
-__global__ __launch_bounds__(1024) void k_gj_panel(double* mat, int64_t ld, int64_t stride, int n,
-                                                   int c, int w, int* piv, int* info) {
+template <int R, int T>
+__global__ __launch_bounds__(T) void k_gj_leaf(double* mat, int64_t ld, int64_t stride, int n, int c,
+                                               int w, int pre_c, int pre_w, int post_c, int post_w,
+                                               int* piv, int* info) {
+    constexpr int NW = T / 64;
     const int64_t b = blockIdx.x;
     double* __restrict__ M = mat + b * stride;
-    int* pv = piv + b * n;
-    const int T = blockDim.x, t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = T >> 6;
-    __shared__ double s_wbest[16];
-    __shared__ int s_widx[16];
-    __shared__ int s_p;
-    __shared__ double s_rowk[GJW], s_rowp[GJW], s_srow[GJW];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    __shared__ double s_best[2][NW];
+    __shared__ int s_idx[2][NW];
+    __shared__ double s_rowp[2][GJW], s_rowk[2][GJW];
+    __shared__ double s_T[GJW][GJW];
+    __shared__ int s_piv[GJW];
+    __shared__ int s_src[2 * GJW], s_dst[2 * GJW];
+    __shared__ int s_cnt, s_sing;
+    if (t == 0) s_sing = 0;
+
     double v[R][GJW];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -213,91 +237,129 @@ __global__ __launch_bounds__(1024) void k_gj_panel(double* mat, int64_t ld, int6
 #pragma unroll
         for (int j = 0; j < GJW; ++j) v[q][j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
     }
-    for (int jj = 0; jj < GJW; ++jj) {
-        if (jj < w) {   // w is uniform across the workgroup
-        const int k = c + jj;
-        // ---- pivot search: max |v(r, jj)| over rows r >= k, first index on ties
+
+    // ---- fused pre-update from the left sibling's pivot block
+    if (pre_w > 0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = t + q * T;
+            if (r >= pre_c && r < pre_c + pre_w) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) s_T[r - pre_c][j] = v[q][j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = t + q * T;
+            if (r < n) {
+                const double* xr = M + (int64_t)r * ld + pre_c;
+                double acc[GJW];
+                const bool inK = (r >= pre_c && r < pre_c + pre_w);
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) acc[j] = inK ? 0.0 : v[q][j];
+#pragma unroll 2
+                for (int kk = 0; kk < pre_w; ++kk) {
+                    {
+                        const double x = xr[kk];
+#pragma unroll
+                        for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) v[q][j] = (j < w) ? acc[j] : 0.0;
+            }
+        }
+    }
+
+    // ---- elimination.  A rolled loop over the pivot columns (the unrolled
+    // form is straight-line code run once per launch, i.e. bound by
+    // instruction fetch); column jj of the register panel is read with
+    // selects.
+#pragma unroll 1
+    for (int jj = 0; jj < w; ++jj) {
+        const int k = c + jj, buf = jj & 1;
+        double colv[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            double f = v[q][0];
+#pragma unroll
+            for (int j = 1; j < GJW; ++j) f = (j == jj) ? v[q][j] : f;
+            colv[q] = f;
+        }
         double best = -1.0;
         int bidx = INT_MAX;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int r = t + q * T;
-            if (r >= k && r < n) {
-                double colv = 0.0;
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) if (j == jj) colv = v[q][j];
-                const double a = fabs(colv);
-                if (a > best) { best = a; bidx = r; }
-            }
+            const double a = fabs(colv[q]);
+            if (r >= k && r < n && a > best) { best = a; bidx = r; }
         }
+        const double mine = best;
+        // wave max, then the lowest lane holding it
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off));
+        {
+            const unsigned long long bal = __ballot(mine == best);
+            const int src = bal ? __ffsll(static_cast<long long>(bal)) - 1 : 0;
+            bidx = __shfl(bidx, src);
+        }
+        if (lane == 0) { s_best[buf][wv] = best; s_idx[buf][wv] = bidx; }
+        __syncthreads();
+        // every wave reduces the NW wave results itself
+        double bb = (lane < NW) ? s_best[buf][lane] : -2.0;
+        int bi = (lane < NW) ? s_idx[buf][lane] : INT_MAX;
+        double mx = bb;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bidx, off);
-            if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+            if (off < NW) mx = fmax(mx, __shfl_xor(mx, off));
         }
-        if (lane == 0) { s_wbest[wv] = best; s_widx[wv] = bidx; }
-        __syncthreads();
-        if (t == 0) {
-            double bb = s_wbest[0]; int bi = s_widx[0];
-            for (int x = 1; x < nw; ++x)
-                if (s_wbest[x] > bb || (s_wbest[x] == bb && s_widx[x] < bi)) { bb = s_wbest[x]; bi = s_widx[x]; }
-            s_p = bi;
-            pv[k] = bi;
-            if (bb == 0.0 && info[b] == 0) info[b] = k + 1;
+        mx = __shfl(mx, 0);
+        {
+            const unsigned long long bal = __ballot(lane < NW && bb == mx);
+            const int src = __ffsll(static_cast<long long>(bal)) - 1;
+            bi = __shfl(bi, src);
         }
-        __syncthreads();
-        const int p = s_p;
-        // ---- exchange rows k and p (panel part through LDS)
+        // no finite candidate (a column of NaN): keep row k, flag singular
+        const int p = (bi == INT_MAX) ? k : bi;
+        // publish rows p and k of the panel
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int r = t + q * T;
-            if (r == k) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) s_rowk[j] = v[q][j];
-            }
             if (r == p) {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) s_rowp[j] = v[q][j];
-            }
-        }
-        __syncthreads();
-        // full-row swap outside the panel; each column stays with one thread
-        if (p != k) {
-            double* rk = M + (int64_t)k * ld;
-            double* rp = M + (int64_t)p * ld;
-            for (int j = t; j < n; j += T) {
-                if (j >= c && j < c + w) continue;
-                const double x = rk[j];
-                rk[j] = rp[j];
-                rp[j] = x;
-            }
-        }
-        // ---- elimination: pivot row is old row p
-        const double pinv = 1.0 / s_rowp[jj];
-        __syncthreads();
-        if (t < GJW) s_srow[t] = s_rowp[t] * pinv;   // scaled pivot row
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r >= n) continue;
-            if (r == p && p != k) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = s_rowk[j];
+                for (int j = 0; j < GJW; ++j) s_rowp[buf][j] = v[q][j];
             }
             if (r == k) {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? pinv : s_srow[j];
-            } else {
-                double f = 0.0;
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) if (j == jj) f = v[q][j];
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? -f * pinv : v[q][j] - f * s_srow[j];
+                for (int j = 0; j < GJW; ++j) s_rowk[buf][j] = v[q][j];
             }
         }
+        if (t == 0) {
+            s_piv[jj] = p;
+            if (!(mx > 0.0) && s_sing == 0) s_sing = k + 1;
+        }
         __syncthreads();
+        const double pinv = 1.0 / s_rowp[buf][jj];
+        double prow[GJW];
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) prow[j] = s_rowp[buf][j] * pinv;   // scaled pivot row
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = t + q * T;
+            if (r == k) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? pinv : prow[j];
+            } else {
+                double f = colv[q];
+                if (r == p) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) v[q][j] = s_rowk[buf][j];
+                    f = s_rowk[buf][jj];
+                }
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? -f * pinv : fma(-f, prow[j], v[q][j]);
+            }
         }
     }
 #pragma unroll
@@ -307,6 +369,96 @@ __global__ __launch_bounds__(1024) void k_gj_panel(double* mat, int64_t ld, int6
 #pragma unroll
             for (int j = 0; j < GJW; ++j)
                 if (j < w) M[(int64_t)r * ld + c + j] = v[q][j];
+        }
+    }
+
+    // ---- composite row interchange for the columns outside the panel.
+    // Slots: s < w is row c+s; w+i is pivot row p_i when p_i lies outside
+    // the panel rows and is its first occurrence.  Wave 0 follows the w
+    // transpositions (slot jj <-> slot of p_jj) on lane-held slot contents.
+    if (wv == 0) {
+        int bslot = 0;
+        int myrow = -1;
+        if (lane < w) {
+            const int p = s_piv[lane];
+            if (p < c + w) bslot = p - c;
+            else {
+                int f = lane;
+                for (int j2 = 0; j2 < lane; ++j2)
+                    if (s_piv[j2] == p) { f = j2; break; }
+                bslot = w + f;
+            }
+            myrow = c + lane;
+        } else if (lane < 2 * w) {
+            myrow = s_piv[lane - w];
+        }
+        int cur = lane;
+        for (int jj = 0; jj < w; ++jj) {
+            const int bs = __builtin_amdgcn_readlane(bslot, jj);
+            const int va = __builtin_amdgcn_readlane(cur, jj);
+            const int vb = __builtin_amdgcn_readlane(cur, bs);
+            if (lane == jj) cur = vb;
+            else if (lane == bs) cur = va;
+        }
+        const int srcrow = __shfl(myrow, cur < 64 ? cur : 0);
+        const bool mv = lane < 2 * w && cur != lane;
+        const unsigned long long bal = __ballot(mv);
+        if (mv) {
+            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+            s_dst[pos] = myrow;
+            s_src[pos] = srcrow;
+        }
+        if (lane == 0) {
+            s_cnt = __popcll(bal);
+            if (s_sing != 0 && info[b] == 0) info[b] = s_sing;
+        }
+        if (lane < w) piv[b * n + c + lane] = s_piv[lane];
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (cnt > 0) {
+        for (int j = t; j < n; j += T) {
+            if (j >= c && j < c + w) continue;
+            double x[2 * GJW];
+#pragma unroll
+            for (int e = 0; e < 2 * GJW; ++e)
+                if (e < cnt) x[e] = M[(int64_t)s_src[e] * ld + j];
+#pragma unroll
+            for (int e = 0; e < 2 * GJW; ++e)
+                if (e < cnt) M[(int64_t)s_dst[e] * ld + j] = x[e];
+        }
+    }
+
+    // ---- fused post-update of columns Jq with this pivot block
+    if (post_w > 0) {
+        __syncthreads();
+        if (t < GJW * GJW) {
+            const int kk = t / GJW, j = t % GJW;
+            s_T[kk][j] = (kk < w && j < post_w) ? M[(int64_t)(c + kk) * ld + post_c + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = t + q * T;
+            if (r < n) {
+                double* xr = M + (int64_t)r * ld + post_c;
+                const double* xk = M + (int64_t)r * ld + c;   // X(r, K), final (re-read: frees v for the swaps)
+                const bool inK = (r >= c && r < c + w);
+                double acc[GJW];
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) acc[j] = (!inK && j < post_w) ? xr[j] : 0.0;
+#pragma unroll 2
+                for (int kk = 0; kk < w; ++kk) {
+                    {
+                        const double x = xk[kk];
+#pragma unroll
+                        for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < GJW; ++j)
+                    if (j < post_w) xr[j] = acc[j];
+            }
         }
     }
 }
@@ -374,15 +526,50 @@ __global__ __launch_bounds__(256) void k_gj_final(const double* mat, double* out
 
 size_t gj_tbuf_elems(int n) { return static_cast<size_t>(n) * n; }
 
-static void gj_panel(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w,
-                     const GjWork& W, hipStream_t s) {
-    ProfScope ps(s, 2.0 * n * static_cast<double>(w) * w * batch, 1);
-    if (n <= 256) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(256), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
-    else if (n <= 512) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(512), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
-    else if (n <= 1024) hipLaunchKernelGGL(k_gj_panel<1>, dim3(batch), dim3(1024), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
-    else if (n <= 2048) hipLaunchKernelGGL(k_gj_panel<2>, dim3(batch), dim3(1024), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
-    else if (n <= 4096) hipLaunchKernelGGL(k_gj_panel<4>, dim3(batch), dim3(1024), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
-    else hipLaunchKernelGGL(k_gj_panel<12>, dim3(batch), dim3(1024), 0, s, mat, ld, stride, n, c, w, W.piv, W.info);
+// Leaf launch: T threads, R = rows per thread (power of two covering n).
+template <int T>
+static void leaf_launch(int R, double* mat, int64_t ld, int64_t stride, int n, int c, int w, int pre_c,
+                        int pre_w, int post_c, int post_w, const GjWork& W, int batch, hipStream_t s) {
+#define SEHF_LEAF(RR)                                                                                   \
+    hipLaunchKernelGGL((k_gj_leaf<RR, T>), dim3(batch), dim3(T), 0, s, mat, ld, stride, n, c, w, pre_c, \
+                       pre_w, post_c, post_w, W.piv, W.info)
+    switch (R) {
+    case 1: SEHF_LEAF(1); break;
+    case 2: SEHF_LEAF(2); break;
+    case 4: SEHF_LEAF(4); break;
+    case 8: SEHF_LEAF(8); break;
+    default: SEHF_LEAF(16); break;
+    }
+#undef SEHF_LEAF
+}
+
+static int g_leaf_threads = [] {   // 0: size-based choice; SEHF_GJ_LEAF_THREADS overrides
+    const char* e = std::getenv("SEHF_GJ_LEAF_THREADS");
+    return e ? std::atoi(e) : 0;
+}();
+
+void gj_set_leaf_threads(int t) { g_leaf_threads = t; }
+
+static void gj_leaf(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w, int pre_c,
+                    int pre_w, int post_c, int post_w, const GjWork& W, hipStream_t s) {
+    double fl = 2.0 * n * static_cast<double>(w) * w;
+    if (pre_w > 0) fl += 2.0 * n * static_cast<double>(pre_w) * w;
+    if (post_w > 0) fl += 2.0 * n * static_cast<double>(w) * post_w;
+    ProfScope ps(s, fl * batch, 1);
+    int T = g_leaf_threads;
+    // registers: about 37 VGPRs per panel row held, 256 at most without spills
+    if (T != 256 && T != 512 && T != 1024) T = (n <= 256) ? 256 : (n <= 512) ? 512 : (n <= 1024) ? 1024 : (n <= 2048) ? 512 : 1024;
+    int R = 1;
+    while (R * T < n) R *= 2;
+    if (R > 16) {
+        T = 1024;
+        R = 1;
+        while (R * T < n) R *= 2;
+        if (R > 16) throw std::runtime_error("Gauss-Jordan leaf: matrix order above 16384");
+    }
+    if (T == 256) leaf_launch<256>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
+    else if (T == 512) leaf_launch<512>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
+    else leaf_launch<1024>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
 }
 
 // Block step of pivot block K = [k0, k0+nk) applied to columns J = [j0, j0+nj):
@@ -404,9 +591,15 @@ static void gj_update(double* mat, int n, int64_t ld, int64_t stride, int batch,
 
 static void gj_rec(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w,
                    const GjWork& W, hipStream_t s) {
-    if (w <= GJW) { gj_panel(mat, n, ld, stride, batch, c, w, W, s); return; }
+    if (w <= GJW) { gj_leaf(mat, n, ld, stride, batch, c, w, 0, 0, 0, 0, W, s); return; }
     int w1 = (w / 2 / GJW) * GJW;
     if (w1 < GJW) w1 = GJW;
+    if (w - w1 <= GJW) {
+        // two leaves: the right one applies both block steps itself
+        gj_leaf(mat, n, ld, stride, batch, c, w1, 0, 0, 0, 0, W, s);
+        gj_leaf(mat, n, ld, stride, batch, c + w1, w - w1, c, w1, c, w1, W, s);
+        return;
+    }
     gj_rec(mat, n, ld, stride, batch, c, w1, W, s);
     gj_update(mat, n, ld, stride, batch, c, w1, c + w1, w - w1, W, s);
     gj_rec(mat, n, ld, stride, batch, c + w1, w - w1, W, s);

@@ -13,6 +13,8 @@
 // the dipole integrands (DipoleMatrixElement.cpp:584-965) and the asymptotic
 // fit sums (AsymptoticAmplitudes.cpp:64-359) into the sweep, so psi is never
 // stored except for the ~200-point fit window.
+#include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -407,6 +409,20 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
     };
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw1 = wall();
+    // Optional progress lines on stderr (SEHF_PROGRESS=<seconds>): the stream
+    // is drained before each line so the reported radial index has finished.
+    static const double prog_every = [] {
+        const char* e = std::getenv("SEHF_PROGRESS");
+        return e ? std::atof(e) : 0.0;
+    }();
+    double prog_last = tw1;
+    auto progress = [&](const char* what, int n, int total) {
+        if (prog_every <= 0.0 || wall() - prog_last < prog_every) return;
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        prog_last = wall();
+        std::fprintf(stderr, "[sehf] batch of %d: %s %d/%d (%.1f s into batch)\n", nb, what, n, total, prog_last - tw0);
+        std::fflush(stderr);
+    };
     // ---------------- forward sweep ----------------
     if (ns0 >= 2) diag_matrix(b.Rp.p, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
     else fill(b.Rp.p, tt * nb, 0.0, st);
@@ -414,6 +430,7 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
         sinv_step(n, b.Sinv.p);
         frp_step(n, b.Sinv.p, b.Rp.p, b.Rc.p);
         std::swap(b.Rp.p, b.Rc.p);
+        progress("forward sweep n =", n, Nr - 2);
         if ((n + 1) % L == 0) {
             const int sidx = (n + 1) / L;
             if (sidx < n_seg)
@@ -524,6 +541,7 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
             }
             prev = Rn;
         }
+        progress("back-propagation segment", sg, n_seg);
         for (int n = n1 - 1; n >= n0; --n) {
             const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
             const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;

@@ -280,7 +280,7 @@ def config(name: str) -> tuple[Problem, np.ndarray]:
     if name == "h2_l8":          # CPU-runnable plain case
         return h2_like(8, 0.01, 30.0, seed=0), k_grid(0.1, 1.0, 10)
     if name == "h2o_l20":        # plain per-ik on one GPU
-        return h2o_like(20, 0.01, 50.0, 5, seed=1), k_grid(0.1, 2.0, 100)
+        return h2o_like(20, 0.01, 50.0, 5, seed=1, allow_closed=True), k_grid(0.1, 2.0, 100)
     if name == "h2o_l30":        # batched scan, closed channels allowed
         return h2o_like(30, 0.01, 60.0, 5, seed=1, allow_closed=True), k_grid(0.05, 2.0, 256)
     if name == "h2o_l20_pol":    # adjoint / polarisation case
