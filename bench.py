@@ -122,7 +122,7 @@ def main():
         print(f"[bench] timed step {i} done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
-    prof = (ctypes.c_double * 6)()
+    prof = (ctypes.c_double * 15)()
     L.sehf_prof_end(prof)
     prof = list(prof)
     energies = B * args.steps

@@ -145,12 +145,19 @@ int  sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len
 /* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
  * fit + dipole, per-batch host tables. */
 int  sehf_scan_stats(const sehf_scan* s, double* t);
+/* Number of energy groups run concurrently by sehf_scan_run (1..4, default
+ * 1 or $SEHF_SCAN_GROUPS): batch i of a run goes to group i % groups; each
+ * group owns a HIP stream and its own batch buffers (device memory grows
+ * linearly with groups) and is driven by its own host thread.  With the
+ * batching it replaces the reference's serial per-ik loop
+ * (src/main.cpp:845-877). */
+int  sehf_scan_set_groups(sehf_scan* s, int groups);
 void sehf_scan_destroy(sehf_scan* s);
 
 /* Kernel timing: between begin and end, every `every`-th FP64 GEMM and
- * Gauss-Jordan panel launch is bracketed by HIP events on its own stream.
- * out[6] = {gemm ms, gemm flop, gemm launches, panel ms, panel flop,
- * panel launches} over the sampled launches. */
+ * Gauss-Jordan leaf launch is bracketed by HIP events on its own stream.
+ * out[15] = five triples {ms, flop, launches} over the sampled launches:
+ * all GEMMs, leaves, GEMMs with K < 64, 64 <= K < 256, K >= 256. */
 int  sehf_prof_begin(int every);
 int  sehf_prof_end(double* out);
 

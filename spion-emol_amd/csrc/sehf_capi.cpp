@@ -53,6 +53,7 @@ struct sehf_ctx {
 };
 
 hipStream_t sehf_ctx_stream(const sehf_ctx* c) { return c->stream; }
+int sehf_ctx_device(const sehf_ctx* c) { return c->device; }
 
 struct sehf_frp {
     sehf_ctx* ctx;

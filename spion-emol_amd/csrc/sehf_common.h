@@ -50,8 +50,8 @@ struct DevArray {
 
 // Gauss-Jordan workspace for `batch` matrices of order n.
 struct GjBuffers {
-    DevArray<int> piv, perm, info;
-    DevArray<double> tbuf;
+    DevArray<int> piv, perm, info, nom, stgi;
+    DevArray<double> tbuf, stg;
     int n = 0, batch = 0;
     void ensure(int n_, int batch_) {
         if (n_ <= n && batch_ <= batch) return;
@@ -60,10 +60,13 @@ struct GjBuffers {
         perm.alloc(static_cast<size_t>(n) * batch);
         info.alloc(static_cast<size_t>(batch));
         tbuf.alloc(gj_tbuf_elems(n) * batch);
+        nom.alloc(gj_nom_elems(n) * batch);
+        stg.alloc(gj_stg_elems() * batch);
+        stgi.alloc(gj_stgi_elems() * batch);
         fill_int(info.p, static_cast<size_t>(batch), 0, nullptr);
         SEHF_HIP_CHECK(hipDeviceSynchronize());
     }
-    GjWork work() const { return GjWork{piv.p, perm.p, tbuf.p, info.p}; }
+    GjWork work() const { return GjWork{piv.p, perm.p, tbuf.p, info.p, nom.p, stg.p, stgi.p}; }
 };
 
 // Throw when any matrix of the last inversion met an exactly-zero pivot

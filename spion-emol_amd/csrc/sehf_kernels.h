@@ -40,12 +40,16 @@ struct GjWork {
     int* perm = nullptr;
     double* tbuf = nullptr;
     int* info = nullptr;
+    int* nom = nullptr;      // leaf nominees (gj_nom_elems per matrix)
+    double* stg = nullptr;   // leaf staging (gj_stg_elems per matrix)
+    int* stgi = nullptr;     // leaf move list (gj_stgi_elems per matrix)
 };
 void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
                 bool symmetrize, const GjWork& w, hipStream_t s);
 size_t gj_tbuf_elems(int n);   // doubles per matrix needed in tbuf
-// Threads per Gauss-Jordan leaf workgroup (256, 512 or 1024; 0 = by order).
-void gj_set_leaf_threads(int t);
+size_t gj_nom_elems(int n);    // ints per matrix: leaf nominees
+size_t gj_stg_elems();         // doubles per matrix: leaf staging
+size_t gj_stgi_elems();        // ints per matrix: leaf move list
 
 // Y[b] = a * X[b] + c * Z[b] + d * I   (n x m column-major blocks; Z may be null).
 void axpbyi(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX, double a,
@@ -71,11 +75,11 @@ void fill_int(int* p, size_t n, int v, hipStream_t s);
 // max_{b,i,j} |X[b](i,j) - X[b](j,i)| into *out (device double), n x n blocks.
 void max_asym(const double* X, int64_t ld, int64_t sX, int n, int batch, double* out, hipStream_t s);
 
-// Per-launch timing of k_dgemm (class 0) and the Gauss-Jordan panel kernel
-// (class 1) with HIP events on the launch stream, for every `every`-th launch
-// of each class.  prof_end waits for the recorded events and writes out[6] =
-// {gemm ms, gemm flop, gemm launches, panel ms, panel flop, panel launches}
-// over the sampled launches.
+// Per-launch timing of k_dgemm and the Gauss-Jordan leaf kernel with HIP
+// events on the launch stream, for every `every`-th launch of each class.
+// prof_end waits for the recorded events and writes out[15] = five triples
+// {ms, flop, launches}: all GEMMs, leaves, GEMMs with K < 64, 64 <= K < 256,
+// K >= 256 (sampling counts per bucket).
 void prof_begin(int every);
 void prof_end(double* out);
 

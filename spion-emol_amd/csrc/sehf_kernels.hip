@@ -16,7 +16,9 @@
 // column-major order, so callers pass and receive ordinary column-major data.
 #include "sehf_kernels.h"
 
+#include <atomic>
 #include <climits>
+#include <mutex>
 #include <cstdlib>
 #include <stdexcept>
 #include <vector>
@@ -133,9 +135,10 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
 // ---------------------------------------------------------------------------
 namespace {
 struct ProfRec { hipEvent_t a, b; double flops; int cls; };
-bool g_prof = false;
+std::atomic<bool> g_prof{false};
+std::mutex g_prof_mu;
 int g_every = 1;
-long long g_count[2] = {0, 0};
+long long g_count[5] = {0, 0, 0, 0, 0};
 std::vector<ProfRec> g_recs;
 std::vector<hipEvent_t> g_pool;
 hipEvent_t take_event() {
@@ -147,22 +150,27 @@ hipEvent_t take_event() {
 }  // namespace
 
 void prof_begin(int every) {
-    g_prof = true;
+    std::lock_guard<std::mutex> lock(g_prof_mu);
     g_every = every > 0 ? every : 1;
-    g_count[0] = g_count[1] = 0;
+    for (auto& x : g_count) x = 0;
     g_recs.clear();
+    g_prof = true;
 }
 
 void prof_end(double* out) {
     g_prof = false;
-    for (int i = 0; i < 6; ++i) out[i] = 0.0;
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    for (int i = 0; i < 15; ++i) out[i] = 0.0;
     for (auto& r : g_recs) {
         (void)hipEventSynchronize(r.b);
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, r.a, r.b);
-        out[3 * r.cls + 0] += ms;
-        out[3 * r.cls + 1] += r.flops;
-        out[3 * r.cls + 2] += 1.0;
+        const int slots[2] = {r.cls == 1 ? 1 : 0, r.cls};   // class 0 sums every GEMM bucket
+        for (int u = 0; u < (r.cls >= 2 ? 2 : 1); ++u) {
+            out[3 * slots[u] + 0] += ms;
+            out[3 * slots[u] + 1] += r.flops;
+            out[3 * slots[u] + 2] += 1.0;
+        }
         g_pool.push_back(r.a);
         g_pool.push_back(r.b);
     }
@@ -174,11 +182,16 @@ struct ProfScope {
     ProfRec r;
     hipStream_t s;
     ProfScope(hipStream_t st, double flops, int cls) : on(g_prof), s(st) {
-        if (on) on = (g_count[cls]++ % g_every) == 0;
+        if (!on) return;
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        on = (g_count[cls]++ % g_every) == 0;
         if (on) { r.a = take_event(); r.b = take_event(); r.flops = flops; r.cls = cls; (void)hipEventRecord(r.a, s); }
     }
     ~ProfScope() {
-        if (on) { (void)hipEventRecord(r.b, s); g_recs.push_back(r); }
+        if (!on) return;
+        std::lock_guard<std::mutex> lock(g_prof_mu);
+        (void)hipEventRecord(r.b, s);
+        g_recs.push_back(r);
     }
 };
 
@@ -187,279 +200,397 @@ void gemm(const Gemm& g, hipStream_t s) {
     GemmK p{g.M, g.N, g.K, g.A, g.lda, g.sA, g.tA, g.B, g.ldb, g.sB, g.tB, g.C, g.ldc, g.sC,
             g.alpha, g.beta, g.zc0, g.zc1};
     dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);
-    ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, 0);
+    // sampled timing buckets by depth: 2 (K < 64), 3 (64 <= K < 256), 4 (K >= 256)
+    ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, g.K < 64 ? 2 : (g.K < 256 ? 3 : 4));
     hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, p);
 }
 
 // ---------------------------------------------------------------------------
 // Gauss-Jordan leaf: pivots c .. c+w-1 (w <= 16) on columns [c, c+w) of the
-// row-major view, one workgroup per matrix, thread t owning rows t, t+T, ...
-// (R rows) of the panel in registers.
+// row-major view, as three kernels that spread each matrix over many
+// workgroups.  Pivot rows come from tournament pivoting (the panel step of
+// communication-avoiding LU):
 //
-// Per pivot column: a max-|.| search (wave shuffles, then one LDS exchange
-// between waves), one LDS broadcast of the pivot row and of row k, and the
-// rank-1 elimination in registers -- two barriers and no global memory
-// traffic.  The row interchanges of the columns outside the panel are
-// applied once at the end as one composite permutation of the <= 2w rows
-// involved (loads of all source rows, then stores).
+//  k_leaf_nominate  grid (row blocks of 256, batch).  Optionally applies the
+//      left sibling's pivot block to this panel first (pre-update).  Every
+//      wave runs partial pivoting on its 64 rows with wave-only traffic (DPP
+//      max, readlanes) and nominates at most w rows; wave 0 repeats the
+//      selection on the 64 nominees' panel values and keeps w per block.
+//  k_leaf_merge     grid (batch), one wave.  Partial pivoting on the block
+//      nominees fixes the w pivot rows in order; inverts the pivot block A_P
+//      (Gauss-Jordan in that order); turns the choice into the LAPACK-style
+//      interchange record piv[c+jj] and the composite move list of the <= 2w
+//      rows involved; stages the old values the apply step must not race on
+//      (moved rows' panel and side-range values, and T = chosen rows of the
+//      side range).
+//  k_leaf_apply     grid (blocks of 256 rows / 256 columns, batch).  Rows:
+//      X(P,K) = A_P^-1, X(O,K) = -A_O A_P^-1 (the w pivot steps restricted to
+//      the panel), and optionally the post-update of the side range Jq with
+//      this pivot block.  Columns outside panel and Jq: the row moves.
 //
-// Optional fused sibling updates (the recursion's block steps for a node of
-// two leaves):
-//   pre  (pre_w > 0): before factoring, apply pivot block Kp = [pre_c,
-//        pre_c+pre_w) to this panel:  X(O,J) += X(O,Kp) T,  X(Kp,J) =
-//        X(Kp,Kp) T,  T = X(Kp,J)_old.
-//   post (post_w > 0): after factoring and the row interchanges, apply this
-//        pivot block K = [c, c+w) to columns Jq = [post_c, post_c+post_w).
+// With the pivot rows fixed the panel result equals w Gauss-Jordan steps on
+// the panel columns; the columns outside the panel only receive the row
+// interchanges (their block updates are the recursion's GEMMs or the fused
+// sibling updates below).
+//
+// Fused sibling updates for a node of two leaves [c, c+w1) and
+// [c+w1, c+w1+w2):  the left leaf's merge stages T = X(K_L, J_R) (after
+// its interchanges) and the right leaf's nominate applies X(O,J_R) +=
+// X(O,K_L) T, X(K_L,J_R) = X(K_L,K_L) T; the right leaf's merge stages
+// T2 = X(K_R, J_L) and its apply performs X(O,J_L) += X(O,K_R) T2,
+// X(K_R,J_L) = X(K_R,K_R) T2.
 // ---------------------------------------------------------------------------
 constexpr int GJW = 16;
+constexpr int LEAF_BLK = 256;          // rows (and columns) per nominate/apply block
+constexpr int STG_D = 4 * GJW * GJW + 2 * 2 * GJW * GJW;   // doubles of staging per matrix
+constexpr int STG_I = 4 * GJW + 4;     // ints of staging per matrix
 
-template <int R, int T>
-__global__ __launch_bounds__(T) void k_gj_leaf(double* mat, int64_t ld, int64_t stride, int n, int c,
-                                               int w, int pre_c, int pre_w, int post_c, int post_w,
-                                               int* piv, int* info) {
-    constexpr int NW = T / 64;
-    const int64_t b = blockIdx.x;
+// staging layout (doubles): Ainv[16][16] | T[16][16] | mvK[32][16] | mvX[32][16]
+template <class P> __device__ __forceinline__ P* stg_ainv(P* s) { return s; }
+template <class P> __device__ __forceinline__ P* stg_T(P* s) { return s + GJW * GJW; }
+template <class P> __device__ __forceinline__ P* stg_mvK(P* s) { return s + 2 * GJW * GJW; }
+template <class P> __device__ __forceinline__ P* stg_mvX(P* s) { return s + 4 * GJW * GJW; }
+// staging ints: dst[32] | src[32] | cnt
+
+__device__ __forceinline__ double lane_read(double x, int l) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// max over the 64 lanes, returned uniform: quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror and row_mirror reduce each 16-lane row; four readlanes
+// combine the rows.
+__device__ __forceinline__ double wave_max(double x) {
+    x = fmax(x, dpp_d<0xB1>(x));
+    x = fmax(x, dpp_d<0x4E>(x));
+    x = fmax(x, dpp_d<0x141>(x));
+    x = fmax(x, dpp_d<0x140>(x));
+    return fmax(fmax(lane_read(x, 0), lane_read(x, 16)), fmax(lane_read(x, 32), lane_read(x, 48)));
+}
+
+// Partial-pivoting selection of up to w rows among the wave's NC candidate
+// slots per lane (row index crow, panel values cv; crow < 0 = empty).  cv is
+// destroyed.  Lane jj < w receives the row chosen for column jj, or -1 when
+// no candidate has a non-zero entry left in that column.
+template <int NC>
+__device__ __forceinline__ int wave_select(double (&cv)[NC][GJW], const int (&crow)[NC], int w, int lane) {
+    unsigned int csel = 0;
+    int chosen = -1;
+#pragma unroll
+    for (int jj = 0; jj < GJW; ++jj) {
+        if (jj < w) {
+            double mine = -1.0;
+            int eb = 0;
+#pragma unroll
+            for (int e = 0; e < NC; ++e) {
+                const double a = fabs(cv[e][jj]);
+                if (crow[e] >= 0 && !((csel >> e) & 1u) && a > mine) { mine = a; eb = e; }
+            }
+            const double m = wave_max(mine);
+            const unsigned long long bal = __ballot(mine == m && mine > 0.0);
+            if (bal != 0ull) {
+                const int L = __ffsll(static_cast<long long>(bal)) - 1;
+                const int ep = __builtin_amdgcn_readlane(eb, L);
+                int rowp = __builtin_amdgcn_readlane(crow[0], L);
+#pragma unroll
+                for (int e = 1; e < NC; ++e) {
+                    const int y = __builtin_amdgcn_readlane(crow[e], L);
+                    rowp = (ep == e) ? y : rowp;
+                }
+                if (lane == L) csel |= 1u << ep;
+                if (lane == jj) chosen = rowp;
+                // pivot row: readlane of every slot, then a uniform select
+                // (selecting first would index the register array dynamically)
+                double prow[GJW];
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) {
+                    if (j >= jj) {
+                        double x = lane_read(cv[0][j], L);
+#pragma unroll
+                        for (int e = 1; e < NC; ++e) {
+                            const double y = lane_read(cv[e][j], L);
+                            x = (ep == e) ? y : x;
+                        }
+                        prow[j] = x;
+                    } else {
+                        prow[j] = 0.0;
+                    }
+                }
+                const double pinv = 1.0 / prow[jj];
+#pragma unroll
+                for (int e = 0; e < NC; ++e) {
+                    if (crow[e] >= 0 && !((csel >> e) & 1u)) {
+                        const double f = cv[e][jj] * pinv;
+#pragma unroll
+                        for (int j = 0; j < GJW; ++j)
+                            if (j > jj) cv[e][j] = fma(-f, prow[j], cv[e][j]);
+                    }
+                }
+            }
+        }
+    }
+    return chosen;
+}
+
+__global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t ld, int64_t stride, int n,
+                                                            int c, int w, int pre_c, int pre_w,
+                                                            const double* stg, int* nom, int nblk) {
+    const int64_t b = blockIdx.y;
     double* __restrict__ M = mat + b * stride;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    __shared__ double s_best[2][NW];
-    __shared__ int s_idx[2][NW];
-    __shared__ double s_rowp[2][GJW], s_rowk[2][GJW];
+    const int r = blockIdx.x * LEAF_BLK + t;
     __shared__ double s_T[GJW][GJW];
-    __shared__ int s_piv[GJW];
-    __shared__ int s_src[2 * GJW], s_dst[2 * GJW];
-    __shared__ int s_cnt, s_sing;
-    if (t == 0) s_sing = 0;
-
-    double v[R][GJW];
+    __shared__ int s_nom[4][GJW];
+    double v[GJW];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int r = t + q * T;
-#pragma unroll
-        for (int j = 0; j < GJW; ++j) v[q][j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
-    }
-
-    // ---- fused pre-update from the left sibling's pivot block
+    for (int j = 0; j < GJW; ++j) v[j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
     if (pre_w > 0) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r >= pre_c && r < pre_c + pre_w) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) s_T[r - pre_c][j] = v[q][j];
-            }
-        }
+        const double* Tp = stg_T(stg + b * STG_D);
+        s_T[t / GJW][t % GJW] = Tp[t];
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r < n) {
-                const double* xr = M + (int64_t)r * ld + pre_c;
-                double acc[GJW];
-                const bool inK = (r >= pre_c && r < pre_c + pre_w);
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) acc[j] = inK ? 0.0 : v[q][j];
-#pragma unroll 2
-                for (int kk = 0; kk < pre_w; ++kk) {
-                    {
-                        const double x = xr[kk];
-#pragma unroll
-                        for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = (j < w) ? acc[j] : 0.0;
-            }
-        }
-    }
-
-    // ---- elimination.  A rolled loop over the pivot columns (the unrolled
-    // form is straight-line code run once per launch, i.e. bound by
-    // instruction fetch); column jj of the register panel is read with
-    // selects.
-#pragma unroll 1
-    for (int jj = 0; jj < w; ++jj) {
-        const int k = c + jj, buf = jj & 1;
-        double colv[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            double f = v[q][0];
-#pragma unroll
-            for (int j = 1; j < GJW; ++j) f = (j == jj) ? v[q][j] : f;
-            colv[q] = f;
-        }
-        double best = -1.0;
-        int bidx = INT_MAX;
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            const double a = fabs(colv[q]);
-            if (r >= k && r < n && a > best) { best = a; bidx = r; }
-        }
-        const double mine = best;
-        // wave max, then the lowest lane holding it
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off));
-        {
-            const unsigned long long bal = __ballot(mine == best);
-            const int src = bal ? __ffsll(static_cast<long long>(bal)) - 1 : 0;
-            bidx = __shfl(bidx, src);
-        }
-        if (lane == 0) { s_best[buf][wv] = best; s_idx[buf][wv] = bidx; }
-        __syncthreads();
-        // every wave reduces the NW wave results itself
-        double bb = (lane < NW) ? s_best[buf][lane] : -2.0;
-        int bi = (lane < NW) ? s_idx[buf][lane] : INT_MAX;
-        double mx = bb;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            if (off < NW) mx = fmax(mx, __shfl_xor(mx, off));
-        }
-        mx = __shfl(mx, 0);
-        {
-            const unsigned long long bal = __ballot(lane < NW && bb == mx);
-            const int src = __ffsll(static_cast<long long>(bal)) - 1;
-            bi = __shfl(bi, src);
-        }
-        // no finite candidate (a column of NaN): keep row k, flag singular
-        const int p = (bi == INT_MAX) ? k : bi;
-        // publish rows p and k of the panel
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r == p) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) s_rowp[buf][j] = v[q][j];
-            }
-            if (r == k) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) s_rowk[buf][j] = v[q][j];
-            }
-        }
-        if (t == 0) {
-            s_piv[jj] = p;
-            if (!(mx > 0.0) && s_sing == 0) s_sing = k + 1;
-        }
-        __syncthreads();
-        const double pinv = 1.0 / s_rowp[buf][jj];
-        double prow[GJW];
-#pragma unroll
-        for (int j = 0; j < GJW; ++j) prow[j] = s_rowp[buf][j] * pinv;   // scaled pivot row
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r == k) {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? pinv : prow[j];
-            } else {
-                double f = colv[q];
-                if (r == p) {
-#pragma unroll
-                    for (int j = 0; j < GJW; ++j) v[q][j] = s_rowk[buf][j];
-                    f = s_rowk[buf][jj];
-                }
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) v[q][j] = (j == jj) ? -f * pinv : fma(-f, prow[j], v[q][j]);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int r = t + q * T;
         if (r < n) {
+            const double* xr = M + (int64_t)r * ld + pre_c;
+            const bool inK = (r >= pre_c && r < pre_c + pre_w);
+            double acc[GJW];
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) acc[j] = inK ? 0.0 : v[j];
+#pragma unroll 4
+            for (int kk = 0; kk < pre_w; ++kk) {
+                const double x = xr[kk];
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) {
+                v[j] = (j < w) ? acc[j] : 0.0;
+                if (j < w) M[(int64_t)r * ld + c + j] = v[j];
+            }
+        }
+    }
+    // wave-local partial pivoting over the wave's rows (r >= c eligible)
+    {
+        double cv[1][GJW];
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) cv[0][j] = v[j];
+        const int crow[1] = {(r >= c && r < n) ? r : -1};
+        const int ch = wave_select<1>(cv, crow, w, lane);
+        if (lane < GJW) s_nom[wv][lane] = (lane < w) ? ch : -1;
+    }
+    __syncthreads();   // nominees listed; pre-updated panel rows visible
+    // block merge: wave 0, one nominee per lane (64 = 4 waves x 16)
+    if (wv == 0) {
+        double cv[1][GJW];
+        int crow[1];
+        crow[0] = s_nom[lane >> 4][lane & 15];
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) cv[0][j] = (crow[0] >= 0 && j < w) ? M[(int64_t)crow[0] * ld + c + j] : 0.0;
+        const int ch = wave_select<1>(cv, crow, w, lane);
+        if (lane < GJW) nom[(b * nblk + blockIdx.x) * GJW + lane] = (lane < w) ? ch : -1;
+    }
+}
+
+// NC = nominee slots per lane: nblk * 16 <= 64 * NC.
+template <int NC>
+__global__ __launch_bounds__(64) void k_leaf_merge(double* mat, int64_t ld, int64_t stride, int n, int c,
+                                                   int w, int xa0, int xw, const int* nom, int nblk,
+                                                   double* stg, int* stgi, int* piv, int* info) {
+    const int64_t b = blockIdx.x;
+    const double* __restrict__ M = mat + b * stride;
+    const int lane = threadIdx.x;
+    double* S = stg + b * STG_D;
+    int* SI = stgi + b * STG_I;
+    double cv[NC][GJW];
+    int crow[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+        const int idx = lane + 64 * e;
+        crow[e] = (idx < nblk * GJW) ? nom[b * nblk * GJW + idx] : -1;
+#pragma unroll
+        for (int j = 0; j < GJW; ++j)
+            cv[e][j] = (crow[e] >= 0 && j < w) ? M[(int64_t)crow[e] * ld + c + j] : 0.0;
+    }
+    int chosen = wave_select<NC>(cv, crow, w, lane);
+    // rank-deficient panel: fill empty steps with rows of [c, c+w) not yet
+    // chosen (one exists for every empty step) and flag the matrix
+    int sing = 0;
+    for (int jj = 0; jj < w; ++jj) {
+        if (__builtin_amdgcn_readlane(chosen, jj) >= 0) continue;
+        if (sing == 0) sing = c + jj + 1;
+        bool fr = lane < w;
+        for (int j2 = 0; j2 < w; ++j2)
+            if (__builtin_amdgcn_readlane(chosen, j2) == c + lane) fr = false;
+        const unsigned long long fb = __ballot(fr);
+        const int rowp = c + __ffsll(static_cast<long long>(fb)) - 1;
+        if (lane == jj) chosen = rowp;
+    }
+    // inverse of the pivot block (row i = panel row of chosen_i), identity
+    // padded to 16 x 16, Gauss-Jordan in pivot order
+    double a[GJW];
+    {
+        const int pr = (lane < w) ? chosen : -1;
+#pragma unroll
+        for (int j = 0; j < GJW; ++j)
+            a[j] = (pr >= 0 && j < w) ? M[(int64_t)pr * ld + c + j] : ((lane == j) ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < GJW; ++k) {
+        double pk[GJW];
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) pk[j] = lane_read(a[j], k);
+        const double pinv = 1.0 / pk[k];
+        if (lane == k) {
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? pinv : pk[j] * pinv;
+        } else {
+            const double f = a[k] * pinv;
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? -f : fma(-f, pk[j], a[j]);
+        }
+    }
+    if (lane < GJW) {
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) stg_ainv(S)[lane * GJW + j] = a[j];
+    }
+    // T = chosen rows x side range [xa0, xa0 + xw) (values before the moves)
+    if (xw > 0 && lane < GJW) {
+        const int pr = (lane < w) ? chosen : -1;
+#pragma unroll
+        for (int j = 0; j < GJW; ++j)
+            stg_T(S)[lane * GJW + j] = (pr >= 0 && j < xw) ? M[(int64_t)pr * ld + xa0 + j] : 0.0;
+    }
+    // interchange record: slot s < w is position c+s, slot w+i is position
+    // chosen_i when it lies below the panel rows; content = original row now
+    // at the slot's position.
+    const int prs = __shfl(chosen, (lane >= w && lane < 2 * w) ? lane - w : 0);
+    int pos = -1;
+    if (lane < w) pos = c + lane;
+    else if (lane < 2 * w) pos = (prs >= c + w) ? prs : -1;
+    int content = pos;
+    int mypiv = 0;
+    for (int jj = 0; jj < w; ++jj) {
+        const int pr = __builtin_amdgcn_readlane(chosen, jj);
+        const unsigned long long hb = __ballot(pos >= 0 && content == pr);
+        const int bs = __ffsll(static_cast<long long>(hb)) - 1;
+        const int pp = __builtin_amdgcn_readlane(pos, bs);
+        const int va = __builtin_amdgcn_readlane(content, jj);
+        const int vb = __builtin_amdgcn_readlane(content, bs);
+        if (lane == jj) { content = vb; mypiv = pp; }
+        else if (lane == bs) content = va;
+    }
+    const bool mv = pos >= 0 && content != pos;
+    const unsigned long long bal = __ballot(mv);
+    if (mv) {
+        const int at = __popcll(bal & ((1ull << lane) - 1ull));
+        SI[at] = pos;                 // dst
+        SI[2 * GJW + at] = content;   // src
+        // old values of the source row: panel and side range
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) {
+            stg_mvK(S)[at * GJW + j] = (j < w) ? M[(int64_t)content * ld + c + j] : 0.0;
+            stg_mvX(S)[at * GJW + j] = (j < xw) ? M[(int64_t)content * ld + xa0 + j] : 0.0;
+        }
+    }
+    if (lane == 0) {
+        SI[4 * GJW] = __popcll(bal);
+        if (sing != 0 && info[b] == 0) info[b] = sing;
+    }
+    if (lane < w) piv[b * n + c + lane] = mypiv;
+}
+
+// post: 1 = update side range Jq = [xa0, xa0+xw) with this pivot block
+// (T2 staged by the merge); 0 = the side range is only moved.
+__global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld, int64_t stride, int n,
+                                                         int c, int w, int xa0, int xw, int post,
+                                                         const double* stg, const int* stgi) {
+    const int64_t b = blockIdx.y;
+    double* __restrict__ M = mat + b * stride;
+    const int t = threadIdx.x;
+    const double* S = stg + b * STG_D;
+    const int* SI = stgi + b * STG_I;
+    __shared__ double s_inv[GJW][GJW];
+    __shared__ double s_T2[GJW][GJW];
+    __shared__ int s_dst[2 * GJW], s_src[2 * GJW];
+    s_inv[t / GJW][t % GJW] = stg_ainv(S)[t];
+    if (post) s_T2[t / GJW][t % GJW] = stg_T(S)[t];
+    if (t < 2 * GJW) { s_dst[t] = SI[t]; s_src[t] = SI[2 * GJW + t]; }
+    const int cnt = SI[4 * GJW];
+    __syncthreads();
+    // ---- rows
+    const int r = blockIdx.x * LEAF_BLK + t;
+    if (r < n) {
+        int e_src = -1;
+#pragma unroll
+        for (int e = 0; e < 2 * GJW; ++e)
+            if (e < cnt && s_dst[e] == r) e_src = e;
+        const bool inK = (r >= c && r < c + w);
+        double o[GJW];
+        if (inK) {
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) o[j] = s_inv[r - c][j];
+        } else {
+            double a[GJW];
+            if (e_src >= 0) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) a[j] = stg_mvK(S)[e_src * GJW + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) a[j] = (j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) o[j] = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < GJW; ++kk) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) o[j] = fma(-a[kk], s_inv[kk][j], o[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < GJW; ++j)
+            if (j < w) M[(int64_t)r * ld + c + j] = o[j];
+        if (post) {
+            double x[GJW];
+            if (inK) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) x[j] = 0.0;
+            } else if (e_src >= 0) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) x[j] = stg_mvX(S)[e_src * GJW + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) x[j] = (j < xw) ? M[(int64_t)r * ld + xa0 + j] : 0.0;
+            }
+#pragma unroll
+            for (int kk = 0; kk < GJW; ++kk) {
+                if (kk < w) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) x[j] = fma(o[kk], s_T2[kk][j], x[j]);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < GJW; ++j)
-                if (j < w) M[(int64_t)r * ld + c + j] = v[q][j];
+                if (j < xw) M[(int64_t)r * ld + xa0 + j] = x[j];
         }
     }
-
-    // ---- composite row interchange for the columns outside the panel.
-    // Slots: s < w is row c+s; w+i is pivot row p_i when p_i lies outside
-    // the panel rows and is its first occurrence.  Wave 0 follows the w
-    // transpositions (slot jj <-> slot of p_jj) on lane-held slot contents.
-    if (wv == 0) {
-        int bslot = 0;
-        int myrow = -1;
-        if (lane < w) {
-            const int p = s_piv[lane];
-            if (p < c + w) bslot = p - c;
-            else {
-                int f = lane;
-                for (int j2 = 0; j2 < lane; ++j2)
-                    if (s_piv[j2] == p) { f = j2; break; }
-                bslot = w + f;
-            }
-            myrow = c + lane;
-        } else if (lane < 2 * w) {
-            myrow = s_piv[lane - w];
-        }
-        int cur = lane;
-        for (int jj = 0; jj < w; ++jj) {
-            const int bs = __builtin_amdgcn_readlane(bslot, jj);
-            const int va = __builtin_amdgcn_readlane(cur, jj);
-            const int vb = __builtin_amdgcn_readlane(cur, bs);
-            if (lane == jj) cur = vb;
-            else if (lane == bs) cur = va;
-        }
-        const int srcrow = __shfl(myrow, cur < 64 ? cur : 0);
-        const bool mv = lane < 2 * w && cur != lane;
-        const unsigned long long bal = __ballot(mv);
-        if (mv) {
-            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-            s_dst[pos] = myrow;
-            s_src[pos] = srcrow;
-        }
-        if (lane == 0) {
-            s_cnt = __popcll(bal);
-            if (s_sing != 0 && info[b] == 0) info[b] = s_sing;
-        }
-        if (lane < w) piv[b * n + c + lane] = s_piv[lane];
-    }
-    __syncthreads();
-    const int cnt = s_cnt;
-    if (cnt > 0) {
-        for (int j = t; j < n; j += T) {
-            if (j >= c && j < c + w) continue;
-            double x[2 * GJW];
+    // ---- columns outside the panel (and outside Jq when post): row moves
+    const int j = blockIdx.x * LEAF_BLK + t;
+    if (cnt > 0 && j < n && !(j >= c && j < c + w) && !(post && j >= xa0 && j < xa0 + xw)) {
+        double x[2 * GJW];
 #pragma unroll
-            for (int e = 0; e < 2 * GJW; ++e)
-                if (e < cnt) x[e] = M[(int64_t)s_src[e] * ld + j];
+        for (int e = 0; e < 2 * GJW; ++e)
+            if (e < cnt) x[e] = M[(int64_t)s_src[e] * ld + j];
 #pragma unroll
-            for (int e = 0; e < 2 * GJW; ++e)
-                if (e < cnt) M[(int64_t)s_dst[e] * ld + j] = x[e];
-        }
-    }
-
-    // ---- fused post-update of columns Jq with this pivot block
-    if (post_w > 0) {
-        __syncthreads();
-        if (t < GJW * GJW) {
-            const int kk = t / GJW, j = t % GJW;
-            s_T[kk][j] = (kk < w && j < post_w) ? M[(int64_t)(c + kk) * ld + post_c + j] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int r = t + q * T;
-            if (r < n) {
-                double* xr = M + (int64_t)r * ld + post_c;
-                const double* xk = M + (int64_t)r * ld + c;   // X(r, K), final (re-read: frees v for the swaps)
-                const bool inK = (r >= c && r < c + w);
-                double acc[GJW];
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) acc[j] = (!inK && j < post_w) ? xr[j] : 0.0;
-#pragma unroll 2
-                for (int kk = 0; kk < w; ++kk) {
-                    {
-                        const double x = xk[kk];
-#pragma unroll
-                        for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < GJW; ++j)
-                    if (j < post_w) xr[j] = acc[j];
-            }
-        }
+        for (int e = 0; e < 2 * GJW; ++e)
+            if (e < cnt) M[(int64_t)s_dst[e] * ld + j] = x[e];
     }
 }
 
@@ -526,50 +657,36 @@ __global__ __launch_bounds__(256) void k_gj_final(const double* mat, double* out
 
 size_t gj_tbuf_elems(int n) { return static_cast<size_t>(n) * n; }
 
-// Leaf launch: T threads, R = rows per thread (power of two covering n).
-template <int T>
-static void leaf_launch(int R, double* mat, int64_t ld, int64_t stride, int n, int c, int w, int pre_c,
-                        int pre_w, int post_c, int post_w, const GjWork& W, int batch, hipStream_t s) {
-#define SEHF_LEAF(RR)                                                                                   \
-    hipLaunchKernelGGL((k_gj_leaf<RR, T>), dim3(batch), dim3(T), 0, s, mat, ld, stride, n, c, w, pre_c, \
-                       pre_w, post_c, post_w, W.piv, W.info)
-    switch (R) {
-    case 1: SEHF_LEAF(1); break;
-    case 2: SEHF_LEAF(2); break;
-    case 4: SEHF_LEAF(4); break;
-    case 8: SEHF_LEAF(8); break;
-    default: SEHF_LEAF(16); break;
-    }
-#undef SEHF_LEAF
-}
+size_t gj_nom_elems(int n) { return static_cast<size_t>((n + LEAF_BLK - 1) / LEAF_BLK) * GJW; }
+size_t gj_stg_elems() { return STG_D; }
+size_t gj_stgi_elems() { return STG_I; }
 
-static int g_leaf_threads = [] {   // 0: size-based choice; SEHF_GJ_LEAF_THREADS overrides
-    const char* e = std::getenv("SEHF_GJ_LEAF_THREADS");
-    return e ? std::atoi(e) : 0;
-}();
-
-void gj_set_leaf_threads(int t) { g_leaf_threads = t; }
-
+// One leaf: pivots [c, c+w).  pre_w > 0: apply the left sibling's block
+// [pre_c, pre_c+pre_w) to this panel first (T staged by that sibling's
+// merge).  The merge stages chosen rows x [xa0, xa0+xw); post = 1 applies
+// this block to those columns, post = 0 leaves them to the sibling.
 static void gj_leaf(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w, int pre_c,
-                    int pre_w, int post_c, int post_w, const GjWork& W, hipStream_t s) {
+                    int pre_w, int xa0, int xw, int post, const GjWork& W, hipStream_t s) {
     double fl = 2.0 * n * static_cast<double>(w) * w;
     if (pre_w > 0) fl += 2.0 * n * static_cast<double>(pre_w) * w;
-    if (post_w > 0) fl += 2.0 * n * static_cast<double>(w) * post_w;
+    if (post) fl += 2.0 * n * static_cast<double>(w) * xw;
     ProfScope ps(s, fl * batch, 1);
-    int T = g_leaf_threads;
-    // registers: about 37 VGPRs per panel row held, 256 at most without spills
-    if (T != 256 && T != 512 && T != 1024) T = (n <= 256) ? 256 : (n <= 512) ? 512 : (n <= 1024) ? 1024 : (n <= 2048) ? 512 : 1024;
-    int R = 1;
-    while (R * T < n) R *= 2;
-    if (R > 16) {
-        T = 1024;
-        R = 1;
-        while (R * T < n) R *= 2;
-        if (R > 16) throw std::runtime_error("Gauss-Jordan leaf: matrix order above 16384");
-    }
-    if (T == 256) leaf_launch<256>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
-    else if (T == 512) leaf_launch<512>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
-    else leaf_launch<1024>(R, mat, ld, stride, n, c, w, pre_c, pre_w, post_c, post_w, W, batch, s);
+    const int nblk = (n + LEAF_BLK - 1) / LEAF_BLK;
+    hipLaunchKernelGGL(k_leaf_nominate, dim3(nblk, batch), dim3(LEAF_BLK), 0, s, mat, ld, stride, n, c, w, pre_c,
+                       pre_w, W.stg, W.nom, nblk);
+    const int nc = (nblk * GJW + 63) / 64;
+#define SEHF_MERGE(NC)                                                                                     \
+    hipLaunchKernelGGL((k_leaf_merge<NC>), dim3(batch), dim3(64), 0, s, mat, ld, stride, n, c, w, xa0, xw, \
+                       W.nom, nblk, W.stg, W.stgi, W.piv, W.info)
+    if (nc <= 1) SEHF_MERGE(1);
+    else if (nc <= 2) SEHF_MERGE(2);
+    else if (nc <= 4) SEHF_MERGE(4);
+    else if (nc <= 8) SEHF_MERGE(8);
+    else if (nc <= 16) SEHF_MERGE(16);
+    else throw std::runtime_error("Gauss-Jordan leaf: matrix order above 16384");
+#undef SEHF_MERGE
+    hipLaunchKernelGGL(k_leaf_apply, dim3(nblk, batch), dim3(LEAF_BLK), 0, s, mat, ld, stride, n, c, w, xa0, xw,
+                       post, W.stg, W.stgi);
 }
 
 // Block step of pivot block K = [k0, k0+nk) applied to columns J = [j0, j0+nj):
@@ -591,13 +708,13 @@ static void gj_update(double* mat, int n, int64_t ld, int64_t stride, int batch,
 
 static void gj_rec(double* mat, int n, int64_t ld, int64_t stride, int batch, int c, int w,
                    const GjWork& W, hipStream_t s) {
-    if (w <= GJW) { gj_leaf(mat, n, ld, stride, batch, c, w, 0, 0, 0, 0, W, s); return; }
+    if (w <= GJW) { gj_leaf(mat, n, ld, stride, batch, c, w, 0, 0, 0, 0, 0, W, s); return; }
     int w1 = (w / 2 / GJW) * GJW;
     if (w1 < GJW) w1 = GJW;
     if (w - w1 <= GJW) {
-        // two leaves: the right one applies both block steps itself
-        gj_leaf(mat, n, ld, stride, batch, c, w1, 0, 0, 0, 0, W, s);
-        gj_leaf(mat, n, ld, stride, batch, c + w1, w - w1, c, w1, c, w1, W, s);
+        // two leaves: the block steps between them are fused into the leaves
+        gj_leaf(mat, n, ld, stride, batch, c, w1, 0, 0, c + w1, w - w1, 0, W, s);
+        gj_leaf(mat, n, ld, stride, batch, c + w1, w - w1, c, w1, c, w1, 1, W, s);
         return;
     }
     gj_rec(mat, n, ld, stride, batch, c, w1, W, s);

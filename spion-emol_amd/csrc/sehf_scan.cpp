@@ -16,6 +16,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -46,6 +49,18 @@ struct Batch {
     GjBuffers gj;
 };
 
+// One energy group: its own stream and batch buffers.  Groups run
+// concurrently (one host thread each) and share the energy-independent
+// device tables read-only.
+struct Group {
+    hipStream_t st = nullptr;
+    Batch bt;
+    double t_stage[8] = {0};
+    ~Group() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
 }  // namespace
 
 struct sehf_scan {
@@ -59,13 +74,17 @@ struct sehf_scan {
     std::vector<double> gA, gS, lA, lS;
     std::vector<char> hasLA, hasLS;
     std::vector<std::vector<double>> A0h, S0h;
-    Batch bt;
+    std::vector<std::unique_ptr<Group>> groups;
+    int n_groups = 1;
+    int ctx_device = 0;
+    std::mutex host_mu;   // lazily computed host tables (stability eigenvalues)
     size_t dev_bytes_shared = 0;
     double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
 };
 
 // context stream accessor (sehf_ctx is defined in sehf_capi.cpp)
 hipStream_t sehf_ctx_stream(const sehf_ctx* c);
+int sehf_ctx_device(const sehf_ctx* c);
 
 namespace {
 
@@ -198,10 +217,12 @@ void build_shared(sehf_scan* s) {
 }
 
 double lmin_A(sehf_scan* s, int n) {
+    std::lock_guard<std::mutex> lock(s->host_mu);
     if (!s->hasLA[n]) { s->lA[n] = sym_min_eig(s->Np, s->A0h[n].data()); s->hasLA[n] = 1; }
     return s->lA[n];
 }
 double lmin_S(sehf_scan* s, int n) {
+    std::lock_guard<std::mutex> lock(s->host_mu);
     if (!s->hasLS[n]) { s->lS[n] = sym_min_eig(s->Np, s->S0h[n].data()); s->hasLS[n] = 1; }
     return s->lS[n];
 }
@@ -224,8 +245,8 @@ void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_ou
     if (meS < thr) { dS_out = thr - meS + Setup::shift_margin; ++shS; }
 }
 
-void alloc_batch(sehf_scan* s, int B, int L, int n_seg, int n_fit) {
-    Batch& b = s->bt;
+void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit) {
+    Batch& b = g.bt;
     if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_seg, 1)) * B * s->Nt * s->Nt &&
         b.win.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np * s->Np)
         return;
@@ -247,7 +268,7 @@ void alloc_batch(sehf_scan* s, int B, int L, int n_seg, int n_fit) {
     b.DRre.alloc(6 * Np * B); b.DRim.alloc(6 * Np * B); b.ore.alloc(Np * s->S->n_occ * B); b.oim.alloc(Np * s->S->n_occ * B);
     b.Dre.alloc(6 * Np * B); b.Dim.alloc(6 * Np * B); b.Ainv.alloc(pp * B); b.Kfit.alloc(pp * B);
     b.dA.alloc(static_cast<size_t>(s->nstab + 1) * B); b.dS.alloc(static_cast<size_t>(s->nstab + 1) * B);
-    b.zeros.alloc(B); fill(b.zeros.p, B, 0.0, s->st); b.cE.alloc(B);
+    b.zeros.alloc(B); fill(b.zeros.p, B, 0.0, g.st); b.cE.alloc(B);
     b.anl.alloc(static_cast<size_t>(s->n_start + 1) * B * Nt);
     b.tab.alloc(6 * Np * B); b.jy.alloc(2 * Np * B);
     b.Jc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.Yc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
@@ -263,11 +284,11 @@ double wall() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out, int out_off) {
+void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_scan_out* out, int out_off) {
     const double tw0 = wall();
     Setup& S = *s->S;
-    hipStream_t st = s->st;
-    Batch& b = s->bt;
+    hipStream_t st = g.st;
+    Batch& b = g.bt;
     const int Np = s->Np, Nf = s->Nf, Nt = s->Nt, Nr = s->Nr, N = Nr - 1, ns = s->nslot, n_occ = S.n_occ;
     const size_t tt = static_cast<size_t>(Nt) * Nt, pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * std::max(Nf, 1);
     const int n_seg = (N + L - 1) / L;
@@ -627,8 +648,8 @@ void run_batch(sehf_scan* s, const double* ks, int nb, int L, sehf_scan_out* out
     SEHF_HIP_CHECK(hipMemcpyAsync(Kh.data(), b.Kfit.p, Kh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw5 = wall();
-    s->t_stage[1] += tw2 - tw1; s->t_stage[2] += tw3 - tw2; s->t_stage[3] += tw4 - tw3;
-    s->t_stage[4] += tw5 - tw4; s->t_stage[5] += tw1 - tw0;
+    g.t_stage[1] += tw2 - tw1; g.t_stage[2] += tw3 - tw2; g.t_stage[3] += tw4 - tw3;
+    g.t_stage[4] += tw5 - tw4; g.t_stage[5] += tw1 - tw0;
     for (int e = 0; e < nb; ++e) {
         const int E_ = out_off + e;
         const double* a = acc.data() + static_cast<size_t>(e) * ns * Np;
@@ -674,6 +695,8 @@ int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
         auto s = std::make_unique<sehf_scan>();
         s->ctx = ctx;
         s->st = sehf_ctx_stream(ctx);
+        s->ctx_device = sehf_ctx_device(ctx);
+        if (const char* e = std::getenv("SEHF_SCAN_GROUPS")) s->n_groups = std::max(1, std::min(4, std::atoi(e)));
         s->S = std::make_unique<Setup>(*p);
         const Setup& S = *s->S;
         s->Np = S.Np; s->Nf = S.Nf; s->Nt = S.Nt; s->Nr = S.Nr; s->n_tr = S.n_tr; s->n_start = S.n_start;
@@ -703,17 +726,47 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
         const int n_seg = (N + L - 1) / L;
         const int n_fit = 200;
         int B = batch > 0 ? std::min(batch, nk) : nk;
-        alloc_batch(s, B, L, n_seg, n_fit);
-        if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
-        for (int e0 = 0; e0 < nk; e0 += B) {
-            const int nb = std::min(B, nk - e0);
-            run_batch(s, k + e0, nb, L, out, e0);
+        const int n_batches = (nk + B - 1) / B;
+        const int G = std::max(1, std::min(s->n_groups, n_batches));
+        while (static_cast<int>(s->groups.size()) < G) {
+            s->groups.emplace_back(new Group());
+            SEHF_HIP_CHECK(hipStreamCreateWithFlags(&s->groups.back()->st, hipStreamNonBlocking));
         }
+        for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit);
+        if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
+        // batch i runs on group i % G; each group is driven by its own host thread
+        std::vector<std::string> errs(G);
+        auto drive = [&](int gi) {
+            try {
+                SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
+                for (int bi = gi; bi < n_batches; bi += G) {
+                    const int e0 = bi * B, nb = std::min(B, nk - e0);
+                    run_batch(s, *s->groups[gi], k + e0, nb, L, out, e0);
+                }
+            } catch (const std::exception& e) {
+                errs[gi] = e.what();
+            }
+        };
+        if (G == 1) {
+            drive(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
+            for (auto& x : th) x.join();
+        }
+        for (int gi = 0; gi < G; ++gi)
+            if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
     });
 }
 
 int sehf_scan_stats(const sehf_scan* s, double* t) {
-    return guarded([&] { for (int i = 0; i < 8; ++i) t[i] = s->t_stage[i]; });
+    return guarded([&] {
+        for (int i = 0; i < 8; ++i) {
+            t[i] = s->t_stage[i];
+            if (i > 0)
+                for (auto& g : s->groups) t[i] += g->t_stage[i];
+        }
+    });
 }
 
 int sehf_prof_begin(int every) {
@@ -722,6 +775,14 @@ int sehf_prof_begin(int every) {
 
 int sehf_prof_end(double* out) {
     return guarded([&] { prof_end(out); });
+}
+
+int sehf_scan_set_groups(sehf_scan* s, int groups) {
+    return guarded([&] {
+        if (groups < 1 || groups > 4)
+            throw std::runtime_error("sehf_scan_set_groups: groups must be in [1, 4] (one stream each)");
+        s->n_groups = groups;
+    });
 }
 
 void sehf_scan_destroy(sehf_scan* s) { delete s; }

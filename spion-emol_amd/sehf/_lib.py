@@ -45,6 +45,7 @@ SIGNATURES = [
     ("sehf_scan_run", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     ("sehf_scan_destroy", None, [_vp]),
     ("sehf_scan_stats", ctypes.c_int, [_vp, _dp]),
+    ("sehf_scan_set_groups", ctypes.c_int, [_vp, ctypes.c_int]),
     ("sehf_prof_begin", ctypes.c_int, [ctypes.c_int]),
     ("sehf_prof_end", ctypes.c_int, [_dp]),
     ("sehf_host_potential", ctypes.c_int, [_vp, ctypes.c_int, _dp]),

@@ -80,8 +80,13 @@ class Scan:
         check(L.sehf_scan_info(h, info))
         self.info = dict(zip(("n_psi", "n_f", "n_tot", "n_transition", "n_start", "n_pts", "Nr", "l_exch"), list(info)))
 
-    def run(self, ks, batch: int = 0, seg_len: int = 0):
+    def run(self, ks, batch: int = 0, seg_len: int = 0, groups: int = 0):
+        """Solve every k.  `batch` energies advance through the radial grid
+        together; `groups` (1..4) batches run concurrently on their own
+        streams (0 keeps the handle's current setting)."""
         L = load()
+        if groups:
+            check(L.sehf_scan_set_groups(self._h, int(groups)), "sehf_scan_set_groups")
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         nk, N, no = len(ks), self.p.n_psi, self.p.n_occ
         o = {"K": np.zeros((nk, N, N)), "A": np.zeros((nk, N, N)), "B": np.zeros((nk, N, N)),

@@ -90,3 +90,19 @@ def test_scan_h2_l8_full(ctx):
     ref = O.solve_scan(p, ks)
     floor = O.noise_floor(p, ks)
     _compare(p, ks, got, ref, label="h2_l8", floor_list=floor)
+
+
+def test_scan_groups_bitwise(ctx):
+    """Energies are independent: batching and concurrent groups (own streams,
+    own host threads) must not change a single bit of any output."""
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=4, dr=0.02, r_max=14.0, n_occ=3, seed=1)
+    ks = np.array([0.45, 0.7, 0.95, 1.1, 1.4])
+    S = scan.Scan(p, ctx)
+    one = S.run(ks, batch=5, groups=1)
+    many = S.run(ks, batch=1, groups=3)
+    two = S.run(ks, batch=2, groups=2)
+    for a, b, c in zip(one, many, two):
+        for key in ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D"):
+            assert np.array_equal(a[key], b[key]), (key, a["k"])
+            assert np.array_equal(a[key], c[key]), (key, a["k"])
