@@ -43,28 +43,23 @@ def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
     return float(sinv + frp + bp + dip + 10 * Nt ** 3)
 
 
-def cpu_baseline(p, k, seconds):
-    """Oracle on the host: per-radial-step cost of Sinv + FRP + BP at full size,
-    energies run concurrently, one thread each; extrapolated to Nr steps."""
+def cpu_baseline(p, ks, n_steps):
+    """The CPU oracle (FP64 restatement of the reference CPU path) on the
+    host: one energy-independent setup, then `cores` energies concurrently
+    (one thread each) run n_steps radial steps of Sinv + FRP + BP at full
+    size; extrapolated to Nr steps per energy."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
-    from concurrent.futures import ThreadPoolExecutor
     cores = max(1, min(os.cpu_count() or 1, 16))
-    n_steps = 1
-    t0 = time.time()
-    per = O.time_steps(p, k, n_steps)          # one energy, one step: calibrates
-    one = float(per.sum())
-    n_steps = max(1, int(seconds / max(one, 1e-3)))
-    n_steps = min(n_steps, 8)
-    with ThreadPoolExecutor(max_workers=cores) as ex:
-        res = list(ex.map(lambda i: O.time_steps(p, k, n_steps), range(cores)))
-    wall = time.time() - t0
-    step_s = float(np.mean([r.sum() for r in res]))
+    sample = np.asarray(ks[:cores], dtype=np.float64)
+    t, wall = O.time_steps_multi(p, sample, n_steps)
+    step_s = float(np.mean(t.sum(axis=1)))
     energy_s = step_s * p.Nr
     return {"value": cores / energy_s, "unit": "energy points/s", "cores": cores, "kind": "port",
-            "sample": f"{n_steps} radial step(s) of Sinv+FRP+BP at full size x {cores} concurrent energies "
-                      f"(one thread each), {wall:.0f} s wall; {step_s:.3f} s per step per energy, "
-                      f"x {p.Nr} steps = {energy_s:.0f} s per energy (extrapolated)"}
+            "sample": f"{n_steps} radial step(s) of Sinv+FRP+BP at full size (N_tot={p.n_tot}) for {cores} "
+                      f"energies run concurrently, one thread each ({wall[1]:.1f} s wall after a {wall[0]:.1f} s "
+                      f"energy-independent setup); {step_s:.2f} s per step per energy x {p.Nr} steps = "
+                      f"{energy_s:.0f} s per energy (extrapolated)"}
 
 
 def main():
@@ -73,11 +68,17 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--groups", type=int, default=1,
+                    help="concurrent energy groups (streams); a step is groups x batch energies")
     ap.add_argument("--config", default="h2o_l30")
     ap.add_argument("--seg-len", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=4, help="radial steps in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-every", type=int, default=97)
+    ap.add_argument("--per-ik", action="store_true",
+                    help="also time the plain per-ik path (batch 1, one group) on one energy")
+    ap.add_argument("--rmax", type=float, default=0.0,
+                    help="profiling aid: shorten the radial grid (same channel sizes); not the metric workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus above 1 is not supported "
@@ -92,6 +93,8 @@ def main():
     torch.cuda.set_device(0)
     t0 = time.time()
     p, ks_all = synth.config(args.config)
+    if args.rmax > 0:
+        p = synth.h2o_like(p.l_cont, p.dr, args.rmax, p.n_occ, seed=1, allow_closed=True)
     t_gen = time.time() - t0
     print(f"[bench] {args.config}: N_psi={p.n_psi} N_f={p.n_f} N_tot={p.n_tot} Nr={p.Nr} "
           f"(inputs generated in {t_gen:.1f} s)", file=sys.stderr, flush=True)
@@ -103,34 +106,43 @@ def main():
           file=sys.stderr, flush=True)
 
     B = args.batch
+    G = max(1, args.groups)
+    E_step = B * G
     nk = len(ks_all)
 
     def batch_k(i):
-        idx = [(i * B + j) % nk for j in range(B)]
+        # a spread of the scan's k values (closed and open channels mixed)
+        idx = [((i * E_step + j) * 37) % nk for j in range(E_step)]
         return ks_all[idx]
 
     L = load()
     for w in range(args.warmup):
-        S.run(batch_k(w), batch=B, seg_len=args.seg_len)
+        S.run(batch_k(w), batch=B, seg_len=args.seg_len, groups=G)
         print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     L.sehf_prof_begin(args.prof_every)
     t0 = time.time()
     outs = []
     for i in range(args.steps):
-        outs.append(S.run(batch_k(args.warmup + i), batch=B, seg_len=args.seg_len))
+        outs.append(S.run(batch_k(args.warmup + i), batch=B, seg_len=args.seg_len, groups=G))
         print(f"[bench] timed step {i} done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
     prof = (ctypes.c_double * 15)()
     L.sehf_prof_end(prof)
     prof = list(prof)
-    energies = B * args.steps
+    energies = E_step * args.steps
     value = energies / elapsed
     mflop = model_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
                                    info["n_transition"], info["n_start"], p.n_occ)
     gemm_tflops = prof[1] / (prof[0] * 1e-3) / 1e12 if prof[0] > 0 else 0.0
     avg_launch_us = prof[0] / max(prof[2], 1) * 1e3
+    buckets = {}
+    for i, nm in enumerate(("gemm_all", "gj_leaf", "gemm_K_lt_64", "gemm_K_64_256", "gemm_K_ge_256")):
+        ms, fl, n = prof[3 * i: 3 * i + 3]
+        if n:
+            buckets[nm] = {"sampled": int(n), "avg_us": ms / n * 1e3, "tflops": fl / (ms * 1e-3) / 1e12,
+                           "est_total_s": ms * args.prof_every / 1e3}
     stages = S.stats()
     for o in outs:
         for r in o:
@@ -145,7 +157,8 @@ def main():
         "config": {"workload": f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), "
                                f"Nr={p.Nr}, dr={p.dr}, k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
                                f"batch {B} energies per step, closed channels allowed",
-                   "batch_energies": B, "seg_len": args.seg_len or "ceil(sqrt(Nr))"},
+                   "batch_energies": B, "groups": G, "energies_per_step": E_step,
+                   "seg_len": args.seg_len or "ceil(sqrt(Nr))"},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
                      "kernel": "k_dgemm", "sampled_launches": int(prof[2]),
@@ -154,9 +167,19 @@ def main():
         "path_tflops": mflop * energies / elapsed / 1e12,
         "path_frac_of_fp64_peak": mflop * energies / elapsed / 1e12 / PEAK_FP64_MFMA_TFLOPS,
         "stage_seconds": stages, "setup_seconds": t_setup,
+        "sampled_kernels": buckets, "prof_every": args.prof_every,
     }
+    if args.rmax > 0:
+        line["config"]["workload"] += f" [profiling grid r_max={args.rmax}, not the metric workload]"
+    if args.per_ik:
+        t0 = time.time()
+        S.run(ks_all[nk // 2: nk // 2 + 1], batch=1, seg_len=args.seg_len, groups=1)
+        torch.cuda.synchronize()
+        t1 = time.time() - t0
+        line["per_ik"] = {"value": 1.0 / t1, "unit": "energy points/s", "seconds": t1,
+                          "batched_over_per_ik": value * t1}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, float(ks_all[nk // 2]), args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(p, batch_k(0), args.cpu_steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 

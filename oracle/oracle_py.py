@@ -146,6 +146,19 @@ def time_steps(p, k, n_steps):
     return t
 
 
+def time_steps_multi(p, ks, n_steps):
+    """Per-step seconds [nk, 3] for nk concurrent energies, plus (setup s, wall s)."""
+    L = lib()
+    ref = ProblemRef(p)
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    t = np.zeros((len(ks), 3))
+    wall = np.zeros(2)
+    rc = L.sehf_oracle_time_steps_multi(ctypes.byref(ref.s), _ptr(ks), len(ks), n_steps, _ptr(t), _ptr(wall))
+    if rc != 0:
+        raise RuntimeError("oracle: " + L.sehf_oracle_last_error().decode())
+    return t, wall
+
+
 def matrix_fn(name, p, ir, shape):
     L = lib()
     ref = ProblemRef(p)

@@ -30,6 +30,8 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -342,6 +344,18 @@ static double w3j(int j1, int j2, int j3, int m1, int m2, int m3) {
     if (m1 + m2 + m3 != 0) return 0.0;
     if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0.0;
     if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
+    if (m1 == 0 && m2 == 0) {
+        // closed form of the m = 0 sum (no cancellation): J = j1+j2+j3 even,
+        // g = J/2: (-1)^g sqrt[(J-2j1)!(J-2j2)!(J-2j3)!/(J+1)!] g!/[(g-j1)!(g-j2)!(g-j3)!]
+        const int J = j1 + j2 + j3;
+        if (J & 1) return 0.0;
+        const int g = J / 2;
+        const auto& lf = log_fact();
+        const long double l = 0.5L * (lf[J - 2 * j1] + lf[J - 2 * j2] + lf[J - 2 * j3] - lf[J + 1]) + lf[g] - lf[g - j1] -
+                              lf[g - j2] - lf[g - j3];
+        const long double v = expl(l);
+        return static_cast<double>((g & 1) ? -v : v);
+    }
     long double mt = 0;
     const long double s = racah_sum<long double>(j1, j2, j3, m1, m2, m3, log_fact(),
                                                  [](long double v) { return expl(v); }, mt);
@@ -539,6 +553,10 @@ static double simpson(const double* f, int n_pts, double h) {
 // ---------------------------------------------------------------------------
 struct GauntRow { std::vector<int> col; std::vector<double> val; };
 
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 class Solver {
 public:
     const sehf_oracle_problem& P;
@@ -576,8 +594,12 @@ public:
         for (int mu = 0; mu < n_mu; ++mu) idx_to_lm(mu, l_mu[mu], m_mu[mu]);
         l_sig.resize(n_sigma);
         for (int s = 0; s < n_sigma; ++s) { int l, m; idx_to_lm(s, l, m); l_sig[s] = l; }
+        const double t0 = now_s();
         build_potential_gaunt();
+        const double t1 = now_s();
         prepare_setup();
+        if (std::getenv("SEHF_ORACLE_VERBOSE"))
+            std::fprintf(stderr, "[oracle] setup: potential Gaunt %.2f s, exchange tables %.2f s\n", t1 - t0, now_s() - t1);
     }
 
     // ---- GauntSparseMatrix::build (GauntSparse.hpp:81-200) ----
@@ -609,7 +631,11 @@ public:
                     const int mu_p = lm_to_idx(l2, m2);
                     for (int l1 = l1_min; l1 <= l1_max; ++l1) {
                         if (((l1 + l_s + l2) & 1) != 0) continue;
+                        // real-harmonic selection rule: G = 0 unless
+                        // |m1| in {|m2|+|m_s|, ||m2|-|m_s||} (the skipped calls are exact zeros)
                         for (int m1 = -l1; m1 <= l1; ++m1) {
+                            const int am = std::abs(m1);
+                            if (am != std::abs(m2) + std::abs(m_s) && am != std::abs(std::abs(m2) - std::abs(m_s))) continue;
                             const int mu = lm_to_idx(l1, m1);
                             if (mu < mu_p) continue;
                             const double G = gaunt_real(l2, m2, l_s, m_s, l1, m1);
@@ -724,6 +750,11 @@ public:
                 int ll, ml; idx_to_lm(lam, ll, ml);
                 for (int s = 0; s < n_sigma; ++s) {
                     int ls, ms; idx_to_lm(s, ls, ms);
+                    // triangle, parity and real-harmonic m rules: the skipped
+                    // calls are exact zeros, so the table is unchanged
+                    if (ll < std::abs(lm - ls) || ll > lm + ls || ((lm + ll + ls) & 1)) continue;
+                    const int am = std::abs(ml);
+                    if (am != std::abs(mm) + std::abs(ms) && am != std::abs(std::abs(mm) - std::abs(ms))) continue;
                     const double val = gaunt_real(lm, mm, ll, ml, ls, ms);
                     if (std::abs(val) > 1e-14) {
                         GauntRow& row = gcomb[static_cast<size_t>(mu) * n_sigma + s];
@@ -911,9 +942,6 @@ struct Scan {
     static void run(const Solver& S, double k, sehf_oracle_out* out);
 };
 
-static double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     const double t0 = now_s();
@@ -1318,6 +1346,54 @@ int sehf_oracle_time_steps(const sehf_oracle_problem* p, double k, int n_steps, 
         }
         const int ns = std::max(1, n1 - n0);
         t[0] = ts / ns; t[1] = tf / ns; t[2] = tb / ns;
+        return 0;
+    } catch (const std::exception& e) {
+        orc::g_err = e.what();
+        return 1;
+    }
+}
+
+int sehf_oracle_time_steps_multi(const sehf_oracle_problem* p, const double* ks, int nk, int n_steps,
+                                 double* t, double* wall) {
+    try {
+        const double t0 = orc::now_s();
+        orc::Solver S(*p);
+        const double t1 = orc::now_s();
+        const int Nt = S.N_tot, Np = S.n_mu;
+        const int n0 = std::max(S.n_start(), 60);
+        const int n1 = std::min(S.Nr - 1, n0 + n_steps);
+        std::vector<std::string> errs(nk);
+#pragma omp parallel for num_threads(nk) schedule(static, 1)
+        for (int e = 0; e < nk; ++e) {
+            try {
+                const double E = 0.5 * ks[e] * ks[e];
+                int shA = 0, shS = 0;
+                double ts = 0, tf = 0, tb = 0;
+                orc::Mat Rprev = orc::identity(Nt);
+                orc::Mat Z(Nt, Np);
+                for (int i = 0; i < Nt; ++i) Z(i, i % Np) = 1.0;
+                orc::Mat Y;
+                for (int n = n0; n < n1; ++n) {
+                    const double a = orc::now_s();
+                    orc::Mat sinv = S.sinv_at(n, E, shA, shS);
+                    const double b = orc::now_s();
+                    orc::Mat r = S.frp_step(n, sinv, Rprev);
+                    const double c = orc::now_s();
+                    Z = orc::mul(r, Z);
+                    S.winv_apply(n, sinv, Z, Y);
+                    const double d = orc::now_s();
+                    ts += b - a; tf += c - b; tb += d - c;
+                    Rprev = r;
+                }
+                const int ns = std::max(1, n1 - n0);
+                t[3 * e + 0] = ts / ns; t[3 * e + 1] = tf / ns; t[3 * e + 2] = tb / ns;
+            } catch (const std::exception& ex) {
+                errs[e] = ex.what();
+            }
+        }
+        for (auto& m : errs) if (!m.empty()) throw std::runtime_error(m);
+        wall[0] = t1 - t0;              // setup (energy independent)
+        wall[1] = orc::now_s() - t1;    // concurrent steps
         return 0;
     } catch (const std::exception& e) {
         orc::g_err = e.what();

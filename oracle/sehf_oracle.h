@@ -70,6 +70,13 @@ int sehf_oracle_solve(const sehf_oracle_problem* p, double k, sehf_oracle_out* o
  * writes per-step seconds: t[0]=Sinv, t[1]=FRP, t[2]=BP(incl. W^-1 apply). */
 int sehf_oracle_time_steps(const sehf_oracle_problem* p, double k, int n_steps, double* t);
 
+/* The same sample for nk energies at once: one energy-independent setup,
+ * then the energies run concurrently (one OpenMP thread each).  t[3*e+i] as
+ * above per energy; wall[0] = setup seconds, wall[1] = wall seconds of the
+ * concurrent steps. */
+int sehf_oracle_time_steps_multi(const sehf_oracle_problem* p, const double* ks, int nk, int n_steps,
+                                 double* t, double* wall);
+
 const char* sehf_oracle_last_error(void);
 
 /* Dense-inverse elimination order: 0 = partial-pivot LU as the reference
