@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--groups", type=int, default=1,
                     help="concurrent energy groups (streams); a step is groups x batch energies")
     ap.add_argument("--config", default="h2o_l30")
@@ -128,7 +128,7 @@ def main():
         print(f"[bench] timed step {i} done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
-    prof = (ctypes.c_double * 15)()
+    prof = (ctypes.c_double * 16)()
     L.sehf_prof_end(prof)
     prof = list(prof)
     energies = E_step * args.steps
@@ -162,7 +162,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
                      "kernel": "k_dgemm", "sampled_launches": int(prof[2]),
-                     "avg_launch_us": avg_launch_us},
+                     "avg_launch_us": avg_launch_us, "event_pair_overhead_us": prof[15] * 1e3},
         "model_tflop_per_energy": mflop / 1e12,
         "path_tflops": mflop * energies / elapsed / 1e12,
         "path_frac_of_fp64_peak": mflop * energies / elapsed / 1e12 / PEAK_FP64_MFMA_TFLOPS,

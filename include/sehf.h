@@ -156,8 +156,10 @@ void sehf_scan_destroy(sehf_scan* s);
 
 /* Kernel timing: between begin and end, every `every`-th FP64 GEMM and
  * Gauss-Jordan leaf launch is bracketed by HIP events on its own stream.
- * out[15] = five triples {ms, flop, launches} over the sampled launches:
- * all GEMMs, leaves, GEMMs with K < 64, 64 <= K < 256, K >= 256. */
+ * out[16] = five triples {ms, flop, launches} over the sampled launches:
+ * all GEMMs, leaves, GEMMs with K < 64, 64 <= K < 256, K >= 256; out[15] =
+ * the event-pair cost of an empty launch on the sampled stream (median of
+ * 32 pairs), already subtracted from every sampled duration. */
 int  sehf_prof_begin(int every);
 int  sehf_prof_end(double* out);
 

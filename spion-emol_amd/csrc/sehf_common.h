@@ -50,7 +50,7 @@ struct DevArray {
 
 // Gauss-Jordan workspace for `batch` matrices of order n.
 struct GjBuffers {
-    DevArray<int> piv, perm, info, nom, stgi;
+    DevArray<int> piv, perm, info, nom, stgi, cnt;
     DevArray<double> tbuf, stg;
     int n = 0, batch = 0;
     void ensure(int n_, int batch_) {
@@ -63,10 +63,12 @@ struct GjBuffers {
         nom.alloc(gj_nom_elems(n) * batch);
         stg.alloc(gj_stg_elems() * batch);
         stgi.alloc(gj_stgi_elems() * batch);
+        cnt.alloc(static_cast<size_t>(batch));
+        fill_int(cnt.p, static_cast<size_t>(batch), 0, nullptr);
         fill_int(info.p, static_cast<size_t>(batch), 0, nullptr);
         SEHF_HIP_CHECK(hipDeviceSynchronize());
     }
-    GjWork work() const { return GjWork{piv.p, perm.p, tbuf.p, info.p, nom.p, stg.p, stgi.p}; }
+    GjWork work() const { return GjWork{piv.p, perm.p, tbuf.p, info.p, nom.p, stg.p, stgi.p, cnt.p}; }
 };
 
 // Throw when any matrix of the last inversion met an exactly-zero pivot

@@ -43,6 +43,7 @@ struct GjWork {
     int* nom = nullptr;      // leaf nominees (gj_nom_elems per matrix)
     double* stg = nullptr;   // leaf staging (gj_stg_elems per matrix)
     int* stgi = nullptr;     // leaf move list (gj_stgi_elems per matrix)
+    int* cnt = nullptr;      // leaf completion counters, one per matrix, zero between leaves
 };
 void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
                 bool symmetrize, const GjWork& w, hipStream_t s);
@@ -77,9 +78,11 @@ void max_asym(const double* X, int64_t ld, int64_t sX, int n, int batch, double*
 
 // Per-launch timing of k_dgemm and the Gauss-Jordan leaf kernel with HIP
 // events on the launch stream, for every `every`-th launch of each class.
-// prof_end waits for the recorded events and writes out[15] = five triples
+// prof_end waits for the recorded events and writes out[16] = five triples
 // {ms, flop, launches}: all GEMMs, leaves, GEMMs with K < 64, 64 <= K < 256,
-// K >= 256 (sampling counts per bucket).
+// K >= 256 (sampling counts per bucket); out[15] = the event-pair cost of an
+// empty launch on the sampled stream (median of 32), already subtracted from
+// every sampled duration.
 void prof_begin(int every);
 void prof_end(double* out);
 

@@ -16,6 +16,7 @@
 // column-major order, so callers pass and receive ordinary column-major data.
 #include "sehf_kernels.h"
 
+#include <algorithm>
 #include <atomic>
 #include <climits>
 #include <mutex>
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
 // Optional per-launch timing (HIP events on the launch stream).
 // ---------------------------------------------------------------------------
 namespace {
-struct ProfRec { hipEvent_t a, b; double flops; int cls; };
+struct ProfRec { hipEvent_t a, b; double flops; int cls; hipStream_t s; };
 std::atomic<bool> g_prof{false};
 std::mutex g_prof_mu;
 int g_every = 1;
@@ -157,14 +158,40 @@ void prof_begin(int every) {
     g_prof = true;
 }
 
+__global__ void k_prof_empty() {}
+
+// Event-pair cost of an empty launch on a stream: what each sampled
+// duration carries on top of its kernel.
+static double event_pair_overhead_ms(hipStream_t s) {
+    std::vector<float> d;
+    hipEvent_t a = take_event(), b = take_event();
+    for (int i = 0; i < 33; ++i) {
+        (void)hipEventRecord(a, s);
+        hipLaunchKernelGGL(k_prof_empty, dim3(1), dim3(64), 0, s);
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (i > 0) d.push_back(ms);
+    }
+    g_pool.push_back(a);
+    g_pool.push_back(b);
+    std::sort(d.begin(), d.end());
+    return d[d.size() / 2];
+}
+
 void prof_end(double* out) {
     g_prof = false;
     std::lock_guard<std::mutex> lock(g_prof_mu);
-    for (int i = 0; i < 15; ++i) out[i] = 0.0;
+    for (int i = 0; i < 16; ++i) out[i] = 0.0;
+    // the sampled launches' stream (any recorded one)
+    const double ovh = g_recs.empty() ? 0.0 : event_pair_overhead_ms(g_recs.front().s);
+    out[15] = ovh;
     for (auto& r : g_recs) {
         (void)hipEventSynchronize(r.b);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        float ms_raw = 0.0f;
+        (void)hipEventElapsedTime(&ms_raw, r.a, r.b);
+        const double ms = std::max(0.0, static_cast<double>(ms_raw) - ovh);
         const int slots[2] = {r.cls == 1 ? 1 : 0, r.cls};   // class 0 sums every GEMM bucket
         for (int u = 0; u < (r.cls >= 2 ? 2 : 1); ++u) {
             out[3 * slots[u] + 0] += ms;
@@ -185,7 +212,7 @@ struct ProfScope {
         if (!on) return;
         std::lock_guard<std::mutex> lock(g_prof_mu);
         on = (g_count[cls]++ % g_every) == 0;
-        if (on) { r.a = take_event(); r.b = take_event(); r.flops = flops; r.cls = cls; (void)hipEventRecord(r.a, s); }
+        if (on) { r.a = take_event(); r.b = take_event(); r.flops = flops; r.cls = cls; r.s = s; (void)hipEventRecord(r.a, s); }
     }
     ~ProfScope() {
         if (!on) return;
@@ -199,9 +226,9 @@ void gemm(const Gemm& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
     GemmK p{g.M, g.N, g.K, g.A, g.lda, g.sA, g.tA, g.B, g.ldb, g.sB, g.tB, g.C, g.ldc, g.sC,
             g.alpha, g.beta, g.zc0, g.zc1};
-    dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);
     // sampled timing buckets by depth: 2 (K < 64), 3 (64 <= K < 256), 4 (K >= 256)
     ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, g.K < 64 ? 2 : (g.K < 256 ? 3 : 4));
+    dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);
     hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, p);
 }
 
@@ -216,13 +243,14 @@ void gemm(const Gemm& g, hipStream_t s) {
 //      wave runs partial pivoting on its 64 rows with wave-only traffic (DPP
 //      max, readlanes) and nominates at most w rows; wave 0 repeats the
 //      selection on the 64 nominees' panel values and keeps w per block.
-//  k_leaf_merge     grid (batch), one wave.  Partial pivoting on the block
-//      nominees fixes the w pivot rows in order; inverts the pivot block A_P
-//      (Gauss-Jordan in that order); turns the choice into the LAPACK-style
-//      interchange record piv[c+jj] and the composite move list of the <= 2w
-//      rows involved; stages the old values the apply step must not race on
-//      (moved rows' panel and side-range values, and T = chosen rows of the
-//      side range).
+//      The block that finishes last (per-matrix completion counter) then
+//      runs the merge with one wave: partial pivoting on the block nominees
+//      fixes the w pivot rows in order; it inverts the pivot block A_P
+//      (Gauss-Jordan in that order), turns the choice into the LAPACK-style
+//      interchange record piv[c+jj] and the composite move list of the
+//      <= 2w rows involved, and stages the old values the apply step must
+//      not race on (moved rows' panel and side-range values, and T = chosen
+//      rows of the side range).
 //  k_leaf_apply     grid (blocks of 256 rows / 256 columns, batch).  Rows:
 //      X(P,K) = A_P^-1, X(O,K) = -A_O A_P^-1 (the w pivot steps restricted to
 //      the panel), and optionally the post-update of the side range Jq with
@@ -342,15 +370,133 @@ __device__ __forceinline__ int wave_select(double (&cv)[NC][GJW], const int (&cr
     return chosen;
 }
 
+// Merge step, run by one wave: partial pivoting on the block nominees fixes
+// the w pivot rows in order; inverts the pivot block A_P (Gauss-Jordan in
+// that order); writes the LAPACK-style interchange record and the composite
+// move list; stages T = chosen rows x side range and the moved rows' old
+// panel / side-range values (all read before anything moves).
+// NC = nominee slots per lane: nblk * 16 <= 64 * NC.
+template <int NC>
+__device__ __forceinline__ void leaf_merge_wave(const double* __restrict__ M, int64_t ld, int n, int c, int w,
+                                                int xa0, int xw, const int* nom_b, int nblk, double* S, int* SI,
+                                                int* piv_b, int* info_b, int lane) {
+    double cv[NC][GJW];
+    int crow[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+        const int idx = lane + 64 * e;
+        crow[e] = (idx < nblk * GJW) ? nom_b[idx] : -1;
+#pragma unroll
+        for (int j = 0; j < GJW; ++j)
+            cv[e][j] = (crow[e] >= 0 && j < w) ? M[(int64_t)crow[e] * ld + c + j] : 0.0;
+    }
+    int chosen = wave_select<NC>(cv, crow, w, lane);
+    // rank-deficient panel: fill empty steps with rows of [c, c+w) not yet
+    // chosen (one exists for every empty step) and flag the matrix
+    int sing = 0;
+    for (int jj = 0; jj < w; ++jj) {
+        if (__builtin_amdgcn_readlane(chosen, jj) >= 0) continue;
+        if (sing == 0) sing = c + jj + 1;
+        bool fr = lane < w;
+        for (int j2 = 0; j2 < w; ++j2)
+            if (__builtin_amdgcn_readlane(chosen, j2) == c + lane) fr = false;
+        const unsigned long long fb = __ballot(fr);
+        const int rowp = c + __ffsll(static_cast<long long>(fb)) - 1;
+        if (lane == jj) chosen = rowp;
+    }
+    // interchange record: slot s < w is position c+s, slot w+i is position
+    // chosen_i when it lies below the panel rows; content = original row now
+    // at the slot's position.
+    const int prs = __shfl(chosen, (lane >= w && lane < 2 * w) ? lane - w : 0);
+    int pos = -1;
+    if (lane < w) pos = c + lane;
+    else if (lane < 2 * w) pos = (prs >= c + w) ? prs : -1;
+    int content = pos;
+    int mypiv = 0;
+    for (int jj = 0; jj < w; ++jj) {
+        const int pr = __builtin_amdgcn_readlane(chosen, jj);
+        const unsigned long long hb = __ballot(pos >= 0 && content == pr);
+        const int bs = __ffsll(static_cast<long long>(hb)) - 1;
+        const int pp = __builtin_amdgcn_readlane(pos, bs);
+        const int va = __builtin_amdgcn_readlane(content, jj);
+        const int vb = __builtin_amdgcn_readlane(content, bs);
+        if (lane == jj) { content = vb; mypiv = pp; }
+        else if (lane == bs) content = va;
+    }
+    const bool mv = pos >= 0 && content != pos;
+    // every value this step reads after selection, issued together: A_P
+    // rows, T = chosen rows x side range, moved rows' old values
+    double a[GJW], tv[GJW], mk[GJW], mx[GJW];
+    {
+        const int pr = (lane < w) ? chosen : -1;
+        const int src = mv ? content : -1;
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) {
+            a[j] = (pr >= 0 && j < w) ? M[(int64_t)pr * ld + c + j] : ((lane == j) ? 1.0 : 0.0);
+            tv[j] = (pr >= 0 && j < xw) ? M[(int64_t)pr * ld + xa0 + j] : 0.0;
+            mk[j] = (src >= 0 && j < w) ? M[(int64_t)src * ld + c + j] : 0.0;
+            mx[j] = (src >= 0 && j < xw) ? M[(int64_t)src * ld + xa0 + j] : 0.0;
+        }
+    }
+    // inverse of the pivot block, identity padded to 16 x 16
+#pragma unroll
+    for (int k = 0; k < GJW; ++k) {
+        double pk[GJW];
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) pk[j] = lane_read(a[j], k);
+        const double pinv = 1.0 / pk[k];
+        if (lane == k) {
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? pinv : pk[j] * pinv;
+        } else {
+            const double f = a[k] * pinv;
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? -f : fma(-f, pk[j], a[j]);
+        }
+    }
+    if (lane < GJW) {
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) {
+            stg_ainv(S)[lane * GJW + j] = a[j];
+            if (xw > 0) stg_T(S)[lane * GJW + j] = tv[j];
+        }
+    }
+    const unsigned long long bal = __ballot(mv);
+    if (mv) {
+        const int at = __popcll(bal & ((1ull << lane) - 1ull));
+        SI[at] = pos;                 // dst
+        SI[2 * GJW + at] = content;   // src
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) {
+            stg_mvK(S)[at * GJW + j] = mk[j];
+            stg_mvX(S)[at * GJW + j] = mx[j];
+        }
+    }
+    if (lane == 0) {
+        SI[4 * GJW] = __popcll(bal);
+        if (sing != 0 && *info_b == 0) *info_b = sing;
+    }
+    if (lane < w) piv_b[c + lane] = mypiv;
+}
+
+// Nominate (all blocks) + merge (the block that finishes last, found with a
+// per-matrix completion counter; no block waits for another).  Each block
+// publishes its nominees and pre-updated panel rows with a device-scope
+// fence before counting in; the last block fences again before reading
+// them and resets the counter for the next leaf.
+template <int NC>
 __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t ld, int64_t stride, int n,
-                                                            int c, int w, int pre_c, int pre_w,
-                                                            const double* stg, int* nom, int nblk) {
+                                                            int c, int w, int pre_c, int pre_w, int xa0,
+                                                            int xw, double* stg, int* stgi, int* nom, int nblk,
+                                                            int* cnt, int* piv, int* info) {
     const int64_t b = blockIdx.y;
     double* __restrict__ M = mat + b * stride;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r = blockIdx.x * LEAF_BLK + t;
     __shared__ double s_T[GJW][GJW];
+    __shared__ double s_rows[LEAF_BLK][GJW + 1];   // the block's panel rows (+1: bank spread)
     __shared__ int s_nom[4][GJW];
+    __shared__ int s_last;
     double v[GJW];
 #pragma unroll
     for (int j = 0; j < GJW; ++j) v[j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
@@ -377,6 +523,8 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t
             }
         }
     }
+#pragma unroll
+    for (int j = 0; j < GJW; ++j) s_rows[t][j] = v[j];
     // wave-local partial pivoting over the wave's rows (r >= c eligible)
     {
         double cv[1][GJW];
@@ -386,125 +534,29 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t
         const int ch = wave_select<1>(cv, crow, w, lane);
         if (lane < GJW) s_nom[wv][lane] = (lane < w) ? ch : -1;
     }
-    __syncthreads();   // nominees listed; pre-updated panel rows visible
-    // block merge: wave 0, one nominee per lane (64 = 4 waves x 16)
+    __syncthreads();
+    // block merge: wave 0, one nominee per lane (64 = 4 waves x 16), values from LDS
     if (wv == 0) {
         double cv[1][GJW];
         int crow[1];
         crow[0] = s_nom[lane >> 4][lane & 15];
+        const int loc = (crow[0] >= 0) ? crow[0] - blockIdx.x * LEAF_BLK : 0;
 #pragma unroll
-        for (int j = 0; j < GJW; ++j) cv[0][j] = (crow[0] >= 0 && j < w) ? M[(int64_t)crow[0] * ld + c + j] : 0.0;
+        for (int j = 0; j < GJW; ++j) cv[0][j] = (crow[0] >= 0) ? s_rows[loc][j] : 0.0;
         const int ch = wave_select<1>(cv, crow, w, lane);
         if (lane < GJW) nom[(b * nblk + blockIdx.x) * GJW + lane] = (lane < w) ? ch : -1;
     }
-}
-
-// NC = nominee slots per lane: nblk * 16 <= 64 * NC.
-template <int NC>
-__global__ __launch_bounds__(64) void k_leaf_merge(double* mat, int64_t ld, int64_t stride, int n, int c,
-                                                   int w, int xa0, int xw, const int* nom, int nblk,
-                                                   double* stg, int* stgi, int* piv, int* info) {
-    const int64_t b = blockIdx.x;
-    const double* __restrict__ M = mat + b * stride;
-    const int lane = threadIdx.x;
-    double* S = stg + b * STG_D;
-    int* SI = stgi + b * STG_I;
-    double cv[NC][GJW];
-    int crow[NC];
-#pragma unroll
-    for (int e = 0; e < NC; ++e) {
-        const int idx = lane + 64 * e;
-        crow[e] = (idx < nblk * GJW) ? nom[b * nblk * GJW + idx] : -1;
-#pragma unroll
-        for (int j = 0; j < GJW; ++j)
-            cv[e][j] = (crow[e] >= 0 && j < w) ? M[(int64_t)crow[e] * ld + c + j] : 0.0;
-    }
-    int chosen = wave_select<NC>(cv, crow, w, lane);
-    // rank-deficient panel: fill empty steps with rows of [c, c+w) not yet
-    // chosen (one exists for every empty step) and flag the matrix
-    int sing = 0;
-    for (int jj = 0; jj < w; ++jj) {
-        if (__builtin_amdgcn_readlane(chosen, jj) >= 0) continue;
-        if (sing == 0) sing = c + jj + 1;
-        bool fr = lane < w;
-        for (int j2 = 0; j2 < w; ++j2)
-            if (__builtin_amdgcn_readlane(chosen, j2) == c + lane) fr = false;
-        const unsigned long long fb = __ballot(fr);
-        const int rowp = c + __ffsll(static_cast<long long>(fb)) - 1;
-        if (lane == jj) chosen = rowp;
-    }
-    // inverse of the pivot block (row i = panel row of chosen_i), identity
-    // padded to 16 x 16, Gauss-Jordan in pivot order
-    double a[GJW];
-    {
-        const int pr = (lane < w) ? chosen : -1;
-#pragma unroll
-        for (int j = 0; j < GJW; ++j)
-            a[j] = (pr >= 0 && j < w) ? M[(int64_t)pr * ld + c + j] : ((lane == j) ? 1.0 : 0.0);
-    }
-#pragma unroll
-    for (int k = 0; k < GJW; ++k) {
-        double pk[GJW];
-#pragma unroll
-        for (int j = 0; j < GJW; ++j) pk[j] = lane_read(a[j], k);
-        const double pinv = 1.0 / pk[k];
-        if (lane == k) {
-#pragma unroll
-            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? pinv : pk[j] * pinv;
-        } else {
-            const double f = a[k] * pinv;
-#pragma unroll
-            for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? -f : fma(-f, pk[j], a[j]);
-        }
-    }
-    if (lane < GJW) {
-#pragma unroll
-        for (int j = 0; j < GJW; ++j) stg_ainv(S)[lane * GJW + j] = a[j];
-    }
-    // T = chosen rows x side range [xa0, xa0 + xw) (values before the moves)
-    if (xw > 0 && lane < GJW) {
-        const int pr = (lane < w) ? chosen : -1;
-#pragma unroll
-        for (int j = 0; j < GJW; ++j)
-            stg_T(S)[lane * GJW + j] = (pr >= 0 && j < xw) ? M[(int64_t)pr * ld + xa0 + j] : 0.0;
-    }
-    // interchange record: slot s < w is position c+s, slot w+i is position
-    // chosen_i when it lies below the panel rows; content = original row now
-    // at the slot's position.
-    const int prs = __shfl(chosen, (lane >= w && lane < 2 * w) ? lane - w : 0);
-    int pos = -1;
-    if (lane < w) pos = c + lane;
-    else if (lane < 2 * w) pos = (prs >= c + w) ? prs : -1;
-    int content = pos;
-    int mypiv = 0;
-    for (int jj = 0; jj < w; ++jj) {
-        const int pr = __builtin_amdgcn_readlane(chosen, jj);
-        const unsigned long long hb = __ballot(pos >= 0 && content == pr);
-        const int bs = __ffsll(static_cast<long long>(hb)) - 1;
-        const int pp = __builtin_amdgcn_readlane(pos, bs);
-        const int va = __builtin_amdgcn_readlane(content, jj);
-        const int vb = __builtin_amdgcn_readlane(content, bs);
-        if (lane == jj) { content = vb; mypiv = pp; }
-        else if (lane == bs) content = va;
-    }
-    const bool mv = pos >= 0 && content != pos;
-    const unsigned long long bal = __ballot(mv);
-    if (mv) {
-        const int at = __popcll(bal & ((1ull << lane) - 1ull));
-        SI[at] = pos;                 // dst
-        SI[2 * GJW + at] = content;   // src
-        // old values of the source row: panel and side range
-#pragma unroll
-        for (int j = 0; j < GJW; ++j) {
-            stg_mvK(S)[at * GJW + j] = (j < w) ? M[(int64_t)content * ld + c + j] : 0.0;
-            stg_mvX(S)[at * GJW + j] = (j < xw) ? M[(int64_t)content * ld + xa0 + j] : 0.0;
-        }
-    }
-    if (lane == 0) {
-        SI[4 * GJW] = __popcll(bal);
-        if (sing != 0 && info[b] == 0) info[b] = sing;
-    }
-    if (lane < w) piv[b * n + c + lane] = mypiv;
+    // count in; the last block of this matrix runs the merge
+    __threadfence();
+    __syncthreads();
+    if (t == 0) s_last = (atomicAdd(cnt + b, 1) == nblk - 1) ? 1 : 0;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (wv == 0)
+        leaf_merge_wave<NC>(M, ld, n, c, w, xa0, xw, nom + b * nblk * GJW, nblk, stg + b * STG_D,
+                            stgi + b * STG_I, piv + b * n, info + b, lane);
+    if (t == 0) cnt[b] = 0;
 }
 
 // post: 1 = update side range Jq = [xa0, xa0+xw) with this pivot block
@@ -520,13 +572,21 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
     __shared__ double s_inv[GJW][GJW];
     __shared__ double s_T2[GJW][GJW];
     __shared__ int s_dst[2 * GJW], s_src[2 * GJW];
+    // own row's panel and side-range values, loaded with the staging data
+    // (replaced from the staging area below when the row receives a move)
+    const int r = blockIdx.x * LEAF_BLK + t;
+    double a0[GJW], x0[GJW];
+#pragma unroll
+    for (int j = 0; j < GJW; ++j) {
+        a0[j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
+        x0[j] = (post && r < n && j < xw) ? M[(int64_t)r * ld + xa0 + j] : 0.0;
+    }
     s_inv[t / GJW][t % GJW] = stg_ainv(S)[t];
     if (post) s_T2[t / GJW][t % GJW] = stg_T(S)[t];
     if (t < 2 * GJW) { s_dst[t] = SI[t]; s_src[t] = SI[2 * GJW + t]; }
     const int cnt = SI[4 * GJW];
     __syncthreads();
     // ---- rows
-    const int r = blockIdx.x * LEAF_BLK + t;
     if (r < n) {
         int e_src = -1;
 #pragma unroll
@@ -544,7 +604,7 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
                 for (int j = 0; j < GJW; ++j) a[j] = stg_mvK(S)[e_src * GJW + j];
             } else {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) a[j] = (j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
+                for (int j = 0; j < GJW; ++j) a[j] = a0[j];
             }
 #pragma unroll
             for (int j = 0; j < GJW; ++j) o[j] = 0.0;
@@ -567,7 +627,7 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
                 for (int j = 0; j < GJW; ++j) x[j] = stg_mvX(S)[e_src * GJW + j];
             } else {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) x[j] = (j < xw) ? M[(int64_t)r * ld + xa0 + j] : 0.0;
+                for (int j = 0; j < GJW; ++j) x[j] = x0[j];
             }
 #pragma unroll
             for (int kk = 0; kk < GJW; ++kk) {
@@ -672,19 +732,17 @@ static void gj_leaf(double* mat, int n, int64_t ld, int64_t stride, int batch, i
     if (post) fl += 2.0 * n * static_cast<double>(w) * xw;
     ProfScope ps(s, fl * batch, 1);
     const int nblk = (n + LEAF_BLK - 1) / LEAF_BLK;
-    hipLaunchKernelGGL(k_leaf_nominate, dim3(nblk, batch), dim3(LEAF_BLK), 0, s, mat, ld, stride, n, c, w, pre_c,
-                       pre_w, W.stg, W.nom, nblk);
     const int nc = (nblk * GJW + 63) / 64;
-#define SEHF_MERGE(NC)                                                                                     \
-    hipLaunchKernelGGL((k_leaf_merge<NC>), dim3(batch), dim3(64), 0, s, mat, ld, stride, n, c, w, xa0, xw, \
-                       W.nom, nblk, W.stg, W.stgi, W.piv, W.info)
-    if (nc <= 1) SEHF_MERGE(1);
-    else if (nc <= 2) SEHF_MERGE(2);
-    else if (nc <= 4) SEHF_MERGE(4);
-    else if (nc <= 8) SEHF_MERGE(8);
-    else if (nc <= 16) SEHF_MERGE(16);
+#define SEHF_NOMINATE(NC)                                                                                   \
+    hipLaunchKernelGGL((k_leaf_nominate<NC>), dim3(nblk, batch), dim3(LEAF_BLK), 0, s, mat, ld, stride, n, c, \
+                       w, pre_c, pre_w, xa0, xw, W.stg, W.stgi, W.nom, nblk, W.cnt, W.piv, W.info)
+    if (nc <= 1) SEHF_NOMINATE(1);
+    else if (nc <= 2) SEHF_NOMINATE(2);
+    else if (nc <= 4) SEHF_NOMINATE(4);
+    else if (nc <= 8) SEHF_NOMINATE(8);
+    else if (nc <= 16) SEHF_NOMINATE(16);
     else throw std::runtime_error("Gauss-Jordan leaf: matrix order above 16384");
-#undef SEHF_MERGE
+#undef SEHF_NOMINATE
     hipLaunchKernelGGL(k_leaf_apply, dim3(nblk, batch), dim3(LEAF_BLK), 0, s, mat, ld, stride, n, c, w, xa0, xw,
                        post, W.stg, W.stgi);
 }
