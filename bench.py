@@ -79,6 +79,9 @@ def main():
                     help="also time the plain per-ik path (batch 1, one group) on one energy")
     ap.add_argument("--rmax", type=float, default=0.0,
                     help="profiling aid: shorten the radial grid (same channel sizes); not the metric workload")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="profiling aid: one device-to-device copy of this many MiB before the timed step, "
+                         "the known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus above 1 is not supported "
@@ -116,6 +119,15 @@ def main():
         return ks_all[idx]
 
     L = load()
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written",
+              file=sys.stderr, flush=True)
     for w in range(args.warmup):
         S.run(batch_k(w), batch=B, seg_len=args.seg_len, groups=G)
         print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
