@@ -43,20 +43,42 @@ namespace {
 
 struct Batch {
     int B = 0;   // capacity
-    DevArray<double> Rp, Rc, U, Ms, Sinv, X, Z, Zt, T, Psi, acc, YJ, YY, Am, Bm, Kd, P, Vpsi, G, Bf;
+    DevArray<double> Rp, Rc, U, Ms, X, Z, Zt, T, Psi, acc, YJ, YY, Am, Bm, Kd, P, Vpsi, G, Bf;
     DevArray<double> ckpt, segR, segS, win, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
-    GjBuffers gj;
+    DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
+    GjBuffers gj;             // inversions on the main stream
+    GjBuffers gj2;            // Schur inversions on the side stream
 };
 
 // One energy group: its own stream and batch buffers.  Groups run
 // concurrently (one host thread each) and share the energy-independent
 // device tables read-only.
 struct Group {
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;    // main stream: forward recursion, K, back-propagation, fit, dipole
+    hipStream_t st2 = nullptr;   // side stream: the Schur inverses Sinv_n, which depend on n and E only
+    hipEvent_t ev_in = nullptr, ev_bwd = nullptr, ev_s[2] = {nullptr, nullptr}, ev_f[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_seg;
     Batch bt;
     double t_stage[8] = {0};
+    void init() {
+        SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_in, &ev_bwd, &ev_s[0], &ev_s[1], &ev_f[0], &ev_f[1]})
+            SEHF_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
+    void ensure_seg_events(int L) {
+        while (static_cast<int>(ev_seg.size()) < L) {
+            hipEvent_t e;
+            SEHF_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev_seg.push_back(e);
+        }
+    }
     ~Group() {
+        for (hipEvent_t e : {ev_in, ev_bwd, ev_s[0], ev_s[1], ev_f[0], ev_f[1]})
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_seg) (void)hipEventDestroy(e);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -255,7 +277,7 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit) {
     b = Batch();
     b.B = B;
     b.Rp.alloc(tt * B); b.Rc.alloc(tt * B); b.U.alloc(tt * B);
-    b.Ms.alloc(pp * B); b.Sinv.alloc(pp * B); b.X.alloc(pf * B);
+    b.Ms.alloc(pp * B); b.X.alloc(pf * B);
     b.Z.alloc(Nt * Np * B); b.Zt.alloc(Nt * Np * B);
     b.T.alloc(pp * B); b.Psi.alloc(pp * B); b.acc.alloc(ns * Np * B);
     b.YJ.alloc(pp * B); b.YY.alloc(pp * B); b.Am.alloc(pp * B); b.Bm.alloc(pp * B); b.Kd.alloc(pp * B);
@@ -274,6 +296,9 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit) {
     b.Jc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.Yc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
     b.coef.alloc(5 * Np * B); b.red.alloc(2 * B); b.nrm.alloc(2 * B);
     b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
+    b.gj2.ensure(static_cast<int>(Np), B);
+    b.SinvR.alloc(2 * pp * B);
+    g.ensure_seg_events(L);
 }
 
 void up(double* d, const std::vector<double>& h, hipStream_t st) {
@@ -396,17 +421,24 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     }
     up(b.dA.p, dA, st); up(b.dS.p, dS, st); up(b.cE.p, cE, st); up(b.anl.p, anl, st);
     up(b.tab.p, tab, st); up(b.jy.p, jy, st); up(b.Jc.p, Jc, st); up(b.Yc.p, Yc, st); up(b.coef.p, coef, st);
+    // the side stream reads the per-energy tables (cE, shifts) just uploaded
+    hipStream_t st2 = g.st2;
+    SEHF_HIP_CHECK(hipEventRecord(g.ev_in, st));
+    SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_in, 0));
     const int64_t sJc = static_cast<int64_t>(n_fit) * Np, sWin = static_cast<int64_t>(n_fit) * pp;
 
     auto gj = [&](double* mat, double* outp, int n, bool sym) {
         gj_inverse(mat, outp, n, n, static_cast<int64_t>(n) * n, nb, sym, b.gj.work(), st);
     };
+    // Sinv_n on the side stream (its own Gauss-Jordan workspace): it needs
+    // only the shared tables and this batch's energies, so it runs ahead of
+    // the forward recursion that consumes it.
     auto sinv_step = [&](int n, double* Sout) {
         const double* a = (n < s->nstab) ? b.dA.p + static_cast<size_t>(n) * nb : b.zeros.p;
         const double* d = (n < s->nstab) ? b.dS.p + static_cast<size_t>(n) * nb : b.zeros.p;
         sinv_matrix(b.Ms.p, s->S0.p + static_cast<size_t>(n) * pp, s->cvd.p + static_cast<size_t>(n) * Np,
-                    s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st);
-        gj(b.Ms.p, Sout, Np, true);
+                    s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st2);
+        gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
     };
     auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout) {
         init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
@@ -447,9 +479,22 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     // ---------------- forward sweep ----------------
     if (ns0 >= 2) diag_matrix(b.Rp.p, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
     else fill(b.Rp.p, tt * nb, 0.0, st);
+    // Sinv_n goes to ring slot n & 1, one step ahead of its consumer: the
+    // side stream waits until FRP(n-2) has read the slot; FRP(n) waits for
+    // Sinv_n.
+    auto sinv_ahead = [&](int n) {
+        const int slot = n & 1;
+        if (n - 2 >= ns0) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
+        sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb);
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
+    };
+    if (ns0 <= Nr - 2) sinv_ahead(ns0);
     for (int n = ns0; n <= Nr - 2; ++n) {
-        sinv_step(n, b.Sinv.p);
-        frp_step(n, b.Sinv.p, b.Rp.p, b.Rc.p);
+        if (n + 1 <= Nr - 2) sinv_ahead(n + 1);
+        const int slot = n & 1;
+        SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
+        frp_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb, b.Rp.p, b.Rc.p);
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
         std::swap(b.Rp.p, b.Rc.p);
         progress("forward sweep n =", n, Nr - 2);
         if ((n + 1) % L == 0) {
@@ -460,6 +505,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
         }
     }
     check_gj_info(b.gj, nb, st, "scan forward sweep");
+    check_gj_info(b.gj2, nb, st2, "scan forward sweep (Schur inverses)");
     const double tw2 = wall();
     // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
     SEHF_HIP_CHECK(hipMemcpyAsync(b.Rc.p, b.Rp.p, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -549,10 +595,17 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     for (int sg = n_seg - 1; sg >= 0; --sg) {
         const int n0 = sg * L, n1 = std::min(n0 + L, N);
         const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
+        // the segment's Schur inverses on the side stream, after the previous
+        // segment's backward products are done with segS
+        if (sg != n_seg - 1) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
+        for (int n = n0; n < n1; ++n) {
+            sinv_step(n, b.segS.p + static_cast<size_t>(n - n0) * nb * pp);
+            SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n - n0], st2));
+        }
         for (int n = n0; n < n1; ++n) {
             double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
             double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
-            sinv_step(n, Sn);
+            SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_seg[n - n0], 0));
             if (n >= ns0) {
                 frp_step(n, Sn, prev, Rn);
             } else if (n == 0) {
@@ -591,8 +644,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
                 fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(n - n_fs) * pp, sWin, b.Psi.p,
                           b.Jc.p + static_cast<size_t>(n - n_fs) * Np, b.Yc.p + static_cast<size_t>(n - n_fs) * Np, sJc, Np, nb, st);
         }
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_bwd, st));   // segS free for the next segment's Sinv
     }
     check_gj_info(b.gj, nb, st, "scan back-propagation");
+    check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
     // ---------------- asymptotic fit ----------------
     fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
@@ -730,7 +785,7 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
         const int G = std::max(1, std::min(s->n_groups, n_batches));
         while (static_cast<int>(s->groups.size()) < G) {
             s->groups.emplace_back(new Group());
-            SEHF_HIP_CHECK(hipStreamCreateWithFlags(&s->groups.back()->st, hipStreamNonBlocking));
+            s->groups.back()->init();
         }
         for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit);
         if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));

@@ -106,3 +106,16 @@ def test_scan_groups_bitwise(ctx):
         for key in ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D"):
             assert np.array_equal(a[key], b[key]), (key, a["k"])
             assert np.array_equal(a[key], c[key]), (key, a["k"])
+
+
+def test_scan_polarisation_cfg4_shape(ctx):
+    """cfg4's extra potential -alpha / (2 (r^2 + r_c^2)^2) (alpha = 9.8,
+    r_c = 1.5) on the diagonal, small grid: parity with the oracle."""
+    import oracle_py as O
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=4, dr=0.02, r_max=14.0, n_occ=3, seed=1, pol=(9.8, 1.5))
+    assert p.pol_kind == 2
+    ks = np.array([0.5, 0.9, 1.3])
+    got = scan.solve_scan(p, ks, ctx=ctx)
+    ref = O.solve_scan(p, ks)
+    _compare(p, ks, got, ref, label="h2o_l4_pol")
