@@ -85,7 +85,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus above 1 is not supported "
-              "(energies shard across GPUs by contiguous k blocks, see DESIGN.md)", file=sys.stderr)
+              "(the design shards energies across GPUs in contiguous k blocks; no collective is needed)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes

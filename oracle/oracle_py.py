@@ -33,7 +33,10 @@ class _Out(ctypes.Structure):
                 ("n_dump_sinv", ctypes.c_int), ("dump_sinv_n", _ip), ("dump_sinv", _dp),
                 ("n_dump_rinv", ctypes.c_int), ("dump_rinv_n", _ip), ("dump_rinv", _dp),
                 ("n_dump_psi", ctypes.c_int), ("dump_psi_n", _ip), ("dump_psi", _dp),
-                ("t_stage", _dp)]
+                ("t_stage", _dp), ("tangent", ctypes.c_int), ("tan_E", ctypes.c_double),
+                ("tan_alpha", ctypes.c_double), ("dK", _dp), ("dA", _dp), ("dB", _dp),
+                ("db_overlap", _dp), ("dd_raw", _dp), ("dD", _dp), ("dump_dpsi", _dp),
+                ("dump_drinv", _dp)]
 
 
 _lib = None
@@ -86,8 +89,11 @@ class ProblemRef:
                           p.pol_rc, 1 if p.allow_closed else 0)
 
 
-def solve(p, k, dump_sinv=(), dump_rinv=(), dump_psi=()):
-    """Run the oracle for one k; returns a dict of numpy outputs."""
+def solve(p, k, dump_sinv=(), dump_rinv=(), dump_psi=(), tangent=None):
+    """Run the oracle for one k; returns a dict of numpy outputs.
+
+    tangent=(tE, talpha) adds the directional derivatives dK, dA, dB,
+    db_overlap, dd_raw, dD along E (Hartree) and the polarisation strength."""
     L = lib()
     ref = ProblemRef(p)
     N, no = p.n_psi, p.n_occ
@@ -108,6 +114,19 @@ def solve(p, k, dump_sinv=(), dump_rinv=(), dump_psi=()):
                len(ds), ds.ctypes.data_as(_ip), _ptr(o["sinv"]),
                len(dri), dri.ctypes.data_as(_ip), _ptr(o["rinv"]),
                len(dp), dp.ctypes.data_as(_ip), _ptr(o["psi"]), _ptr(o["t_stage"]))
+    if tangent is not None:
+        o.update({"dK": np.zeros((N, N), order="F"), "dA": np.zeros((N, N), order="F"),
+                  "dB": np.zeros((N, N), order="F"), "db_overlap": np.zeros((N, no), order="F"),
+                  "dd_raw": np.zeros((6, N)), "dD": np.zeros((6, N), dtype=np.complex128)})
+        out.tangent = 1
+        out.tan_E, out.tan_alpha = float(tangent[0]), float(tangent[1])
+        out.dK, out.dA, out.dB = _ptr(o["dK"]), _ptr(o["dA"]), _ptr(o["dB"])
+        out.db_overlap, out.dd_raw = _ptr(o["db_overlap"]), _ptr(o["dd_raw"])
+        out.dD = o["dD"].ctypes.data_as(_dp)
+        o["dpsi"] = np.zeros((len(dp), N, N))
+        out.dump_dpsi = _ptr(o["dpsi"])
+        o["drinv"] = np.zeros((len(dri), p.n_tot, p.n_tot))
+        out.dump_drinv = _ptr(o["drinv"])
     rc = L.sehf_oracle_solve(ctypes.byref(ref.s), ctypes.c_double(k), ctypes.byref(out))
     if rc != 0:
         raise RuntimeError("oracle: " + L.sehf_oracle_last_error().decode())
@@ -115,6 +134,10 @@ def solve(p, k, dump_sinv=(), dump_rinv=(), dump_psi=()):
     o["sinv"] = np.ascontiguousarray(o["sinv"].transpose(0, 2, 1))
     o["rinv"] = np.ascontiguousarray(o["rinv"].transpose(0, 2, 1))
     o["psi"] = np.ascontiguousarray(o["psi"].transpose(0, 2, 1))
+    if "drinv" in o:
+        o["drinv"] = np.ascontiguousarray(o["drinv"].transpose(0, 2, 1))
+    if "dpsi" in o:
+        o["dpsi"] = np.ascontiguousarray(o["dpsi"].transpose(0, 2, 1))
     return o
 
 

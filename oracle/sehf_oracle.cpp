@@ -508,6 +508,17 @@ static double riccati_y(int l, double x) {
     if (x < 1e-300) throw std::runtime_error("riccati_y called at x ~ 0");
     return x * sph_y(l, x);
 }
+// d/dx of the Riccati forms u_l = x f_l:  u_l' = u_{l-1} - (l/x) u_l, with
+// u_{-1} = cos x (j) and sin x (y, where u_0 = -cos x).
+static double riccati_jp(int l, double x) {
+    if (l == 0) return std::cos(x);
+    if (x < 1e-300) return 0.0;
+    return riccati_j(l - 1, x) - static_cast<double>(l) / x * riccati_j(l, x);
+}
+static double riccati_yp(int l, double x) {
+    if (l == 0) return std::sin(x);
+    return riccati_y(l - 1, x) - static_cast<double>(l) / x * riccati_y(l, x);
+}
 
 // ---------------------------------------------------------------------------
 // Quadrature helpers (DipoleMatrixElement.cpp:49-74, :142-192).
@@ -713,6 +724,22 @@ public:
             if (r < 1.0e-6) return 0.0;
             const double q = r * r + P.pol_rc * P.pol_rc;
             return -P.pol_alpha / (2.0 * q * q);
+        }
+        return 0.0;
+    }
+    // dV_pol/d alpha at potential index n (both forms are linear in alpha;
+    // V is zero at r = 0)
+    double pol_unit_at(int n) const {
+        const double r = n * dr;
+        if (!(r > 1e-14) || r < 1.0e-6) return 0.0;
+        if (P.pol_kind == 1) {
+            const double u = r / 1.5, u6 = u * u * u * u * u * u;
+            const double damp = 1.0 - std::exp(-u6);
+            return -0.5 * damp * damp / (r * r * r * r);
+        }
+        if (P.pol_kind == 2) {
+            const double q = r * r + P.pol_rc * P.pol_rc;
+            return -1.0 / (2.0 * q * q);
         }
         return 0.0;
     }
@@ -923,6 +950,28 @@ public:
         }
         return R;
     }
+    // Tangent of analytic_init along (E, alpha): dk = tE / k, and the
+    // potential-index weights w_n = tE - tA * dV_pol/dalpha enter M_n.
+    Mat analytic_init_tangent(int n, double E, double tE, double tA) const {
+        Mat dR(N_tot, N_tot);
+        if (n <= 0) return dR;
+        const double h = dr, h2_6 = h * h / 6.0;
+        const double k = std::sqrt(2.0 * E), dk = tE / k;
+        const double r_n = dr + n * h, r_np1 = dr + (n + 1) * h;
+        const double wn = tE - tA * pol_unit_at(n), wn1 = tE - tA * pol_unit_at(n + 1);
+        const Mat Vn = potential(n), Vnp1 = potential(n + 1);
+        for (int mu = 0; mu < n_mu; ++mu) {
+            const int l = l_mu[mu];
+            const double Mn = 1.0 + h2_6 * (E - Vn(mu, mu));
+            const double Mnp1 = 1.0 + h2_6 * (E - Vnp1(mu, mu));
+            const double yn = riccati_j(l, k * r_n), ynp1 = riccati_j(l, k * r_np1);
+            const double dyn = riccati_jp(l, k * r_n) * r_n * dk, dynp1 = riccati_jp(l, k * r_np1) * r_np1 * dk;
+            const double a = Mn * yn, b = Mnp1 * ynp1;
+            const double da = h2_6 * wn * yn + Mn * dyn, db = h2_6 * wn1 * ynp1 + Mnp1 * dynp1;
+            if (std::abs(b) > 1e-300) dR(mu, mu) = (da * b - a * db) / (b * b);
+        }
+        return dR;   // the f block does not depend on E or alpha
+    }
     // One FRP step: Rinv_n = sym(inv(12 W^{-1} - 10 I - Rinv_prev)).
     Mat frp_step(int n, const Mat& Sinv, const Mat& Rprev) const {
         Mat Y;
@@ -949,27 +998,55 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     const int Np = S.n_mu, Nf = S.N_f, Nt = S.N_tot, Nr = S.Nr, n_occ = S.n_occ;
     const double h = S.dr, h2_6 = h * h / 6.0;
     double ts[8] = {0};
+    // Tangent along (E, alpha) = (tE, tA) when requested.  Every E- or
+    // alpha-dependence of A_n = I + h^2/6 (E - V_n) is a multiple of I,
+    // dA_n = h^2/6 w_n I with w_n = tE - tA dV_pol/dalpha(n); a stability
+    // shift rebuilds A (or S) as M - lambda_min(M) I + const, which removes
+    // that dependence (sig[n] = 0).  The Riccati functions move with
+    // dk = tE / k.
+    const bool tan = out->tangent != 0;
+    const double tE = out->tan_E, tA = out->tan_alpha, dk = tE / k;
+    auto wn = [&](int n) { return tE - tA * S.pol_unit_at(n); };
     // ---- Sinv (SchurInverter::build, SchurInverter.cpp:465-473) ----
     std::vector<Mat> sinv(Nr);
+    std::vector<char> sig(Nr, 1);
     int shA = 0, shS = 0;
-    for (int n = 0; n < Nr; ++n) sinv[n] = S.sinv_at(n, E, shA, shS);
+    for (int n = 0; n < Nr; ++n) {
+        const int a0 = shA, s0 = shS;
+        sinv[n] = S.sinv_at(n, E, shA, shS);
+        sig[n] = (shA == a0 && shS == s0) ? 1 : 0;
+    }
     const double t1 = now_s(); ts[1] = t1 - t0;
     // ---- FRP (ForwardRPropagator::run, ForwardRPropagator.cpp:136-401) ----
     const int ns = S.n_start();
-    std::vector<Mat> rinv(Nr);
+    std::vector<Mat> rinv(Nr), drinv(tan ? Nr : 0);
     Mat Rprev(Nt, Nt);
     for (int n = 0; n < ns; ++n) {
         rinv[n] = S.analytic_init(n, E);
+        if (tan) drinv[n] = S.analytic_init_tangent(n, E, tE, tA);
         if (n == ns - 1) Rprev = rinv[n];
     }
+    Mat Epsi(Nt, Np);
+    for (int i = 0; i < Np; ++i) Epsi(i, i) = 1.0;
     for (int n = ns; n < Nr; ++n) {
         rinv[n] = S.frp_step(n, sinv[n], Rprev);
         Rprev = rinv[n];
+        if (tan) {
+            // dRinv_n = Rinv_n (dRinv_{n-1} + 12 sig h^2/6 w_n G G^T) Rinv_n,
+            // G = W_n^{-1}[:, psi] (the derivative of 12 W^{-1} - 10 I - Rinv_{n-1})
+            Mat G;
+            S.winv_apply(n, sinv[n], Epsi, G);
+            Mat H = (n >= 1) ? drinv[n - 1] : Mat(Nt, Nt);
+            gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, transpose(G), 1.0, H);
+            const Mat T = mul(H, rinv[n]);
+            drinv[n] = mul(rinv[n], T);
+            symmetrize(drinv[n]);
+        }
     }
     const double t2 = now_s(); ts[2] = t2 - t1;
     // ---- KMatrixExtractor::extract (KMatrixExtractor.cpp:75-198) ----
     const int nM = Nr - 2;
-    Mat Kk;
+    Mat Kk, dKk;
     double unit_err_kme = 0.0;
     {
         const Mat R = inverse_general(rinv[nM]);
@@ -1008,6 +1085,32 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
         Mat lu = Bm; std::vector<int> perm; lu_partial(lu, perm);
         Kk = Am; lu_solve(lu, perm, Kk);
         symmetrize(Kk);
+        if (tan) {
+            // S_schur = ((Rinv_{nM})_psi,psi)^{-1}, so dS = -S (dRinv)_psi,psi S;
+            // dK = sym(Bm^{-1} (dAm - dBm K)).
+            Mat dRpp(Np, Np);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dRpp(i, j) = drinv[nM](i, j);
+            Mat dSs(Np, Np);
+            gemm(-1.0, mul(Ssch, dRpp), Ssch, 0.0, dSs);
+            Mat dAm(Np, Np), dBm(Np, Np);
+            const double dW = h2_12 * 2.0 * k * dk;
+            for (int j = 0; j < Np; ++j) {
+                const int l = S.l_mu[j];
+                const double dJin = riccati_jp(l, k * r_in) * r_in * dk, dJout = riccati_jp(l, k * r_out) * r_out * dk;
+                const double dYin = riccati_yp(l, k * r_in) * r_in * dk, dYout = riccati_yp(l, k * r_out) * r_out * dk;
+                const double WJin = Win[j] * Jin[j], dWJin = dW * Jin[j] + Win[j] * dJin;
+                const double WYin = Win[j] * Yin[j], dWYin = dW * Yin[j] + Win[j] * dYin;
+                for (int i = 0; i < Np; ++i) {
+                    dAm(i, j) = ((i == j) ? dW * Jout[j] + Wout[j] * dJout : 0.0) - dSs(i, j) * WJin - Ssch(i, j) * dWJin;
+                    dBm(i, j) = dSs(i, j) * WYin + Ssch(i, j) * dWYin - ((i == j) ? dW * Yout[j] + Wout[j] * dYout : 0.0);
+                }
+            }
+            Mat rhs = dAm;
+            gemm(-1.0, dBm, Kk, 1.0, rhs);
+            lu_solve(lu, perm, rhs);
+            symmetrize(rhs);
+            dKk = rhs;
+        }
         // S = (I + iK)(I - iK)^{-1}; unitarity diagnostic
         CMat lhs(Np, Np), rhs(Np, Np);
         for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) {
@@ -1030,41 +1133,103 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             psiN(i, j) = ((i == j) ? riccati_j(l, k * rN) : 0.0) + riccati_y(l, k * rN) * Kk(i, j);
         }
     }
+    Mat dpsiN;
+    if (tan) {
+        const double rN = h + (Nr - 1) * h;
+        dpsiN.resize(Np, Np);
+        for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) {
+            const int l = S.l_mu[i];
+            dpsiN(i, j) = ((i == j) ? riccati_jp(l, k * rN) * rN * dk : 0.0) + riccati_yp(l, k * rN) * rN * dk * Kk(i, j) +
+                          riccati_y(l, k * rN) * dKk(i, j);
+        }
+    }
     const double t3 = now_s(); ts[3] = t3 - t2;
     // ---- BackPropagator::run (BackPropagator.cpp:199-758) ----
     const int N = Nr - 1;
     const int n_main_hi = std::min(Nr - 1, std::max(1, S.n_tr - 1));   // main.cpp:1267-1270
     const int n_asym = 300, asym_off = N - n_asym + 1;                  // main.cpp:1335
     std::vector<Mat> psi_keep(n_main_hi + 1), psi_asym(n_asym);
+    std::vector<Mat> dpsi_keep(tan ? n_main_hi + 1 : 0), dpsi_asym(tan ? n_asym : 0);
     auto store = [&](int n, const Mat& psi) {
         if (n >= asym_off && n <= N) psi_asym[n - asym_off] = psi;
         if (n >= 0 && n <= n_main_hi) psi_keep[n] = psi;
     };
+    auto dstore = [&](int n, const Mat& dpsi) {
+        if (n >= asym_off && n <= N) dpsi_asym[n - asym_off] = dpsi;
+        if (n >= 0 && n <= n_main_hi) dpsi_keep[n] = dpsi;
+    };
     store(N, psiN);
-    Mat Z(Nt, Np);
+    if (tan) dstore(N, dpsiN);
+    // Z at the outer boundary: Z_psi = psi + h^2/6 (E psi - V_N psi) + B_N f_N
+    // with f_N = -D_N^{-1} B_N^T psi, Z_f = 0.  Its tangent is the same map on
+    // dpsi_N plus h^2/6 w_N psi_N.
+    Mat Z(Nt, Np), dZ;
     {
         const Mat BN = S.Bmat(N);
         const std::vector<double> DiN = S.Dinv(N);
-        Mat fN = mul(transpose(BN), psiN);
-        for (int j = 0; j < Np; ++j) for (int f = 0; f < Nf; ++f) fN(f, j) = -(DiN[f] * fN(f, j));
         const Mat VN = S.potential(N);
-        Mat Vpsi(Np, Np); gemm(h2_6, VN, psiN, 0.0, Vpsi);
-        Mat Bf = mul(BN, fN);
-        for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i)
-            Z(i, j) = ((psiN(i, j) + h2_6 * E * psiN(i, j)) - Vpsi(i, j)) + Bf(i, j);
+        auto zmap = [&](const Mat& ps, Mat& Zo) {
+            Mat fN = mul(transpose(BN), ps);
+            for (int j = 0; j < Np; ++j) for (int f = 0; f < Nf; ++f) fN(f, j) = -(DiN[f] * fN(f, j));
+            Mat Vpsi(Np, Np); gemm(h2_6, VN, ps, 0.0, Vpsi);
+            Mat Bf = mul(BN, fN);
+            Zo.resize(Nt, Np);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i)
+                Zo(i, j) = ((ps(i, j) + h2_6 * E * ps(i, j)) - Vpsi(i, j)) + Bf(i, j);
+        };
+        zmap(psiN, Z);
+        if (tan) {
+            zmap(dpsiN, dZ);
+            const double c = h2_6 * wn(N);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dZ(i, j) += c * psiN(i, j);
+        }
     }
-    Mat Y;
+    Mat Y, Yd;
     for (int n = N - 1; n >= 0; --n) {
+        const Mat Zold = tan ? Z : Mat();
         Z = mul(rinv[n], Z);
         S.winv_apply(n, sinv[n], Z, Y);
         Mat psi(Np, Np);
         for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) psi(i, j) = Y(i, j);
         store(n, psi);
+        if (tan) {
+            // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start,
+            // dRinv_n = Rinv_n H_n Rinv_n with H_n = dRinv_{n-1} + s_n G G^T,
+            // so dRinv_n Z_{n+1} = Rinv_n H_n Z_n and G^T Z_n = psi_n:
+            //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + s_n G psi_n).
+            // This grouping never multiplies by dRinv_n itself, which grows
+            // like 1/delta^2 where Rinv_n passes near a pole (a near-singular
+            // ratio); its rounding error then lies along the pole direction,
+            // which Rinv_{n+1} maps to O(delta).
+            Mat br = dZ;
+            if (n >= ns) {
+                gemm(1.0, drinv[n - 1], Z, 1.0, br);
+                Mat G;
+                S.winv_apply(n, sinv[n], Epsi, G);
+                gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, psi, 1.0, br);
+                dZ = mul(rinv[n], br);
+            } else {
+                dZ = mul(rinv[n], br);
+                gemm(1.0, drinv[n], Zold, 1.0, dZ);
+            }
+            // dpsi = (W^{-1} dZ')_psi + (dW^{-1} Z')_psi, dW^{-1} = -sig h^2/6 w G G^T
+            // and G^T Z' = psi, G_psi = Sinv
+            S.winv_apply(n, sinv[n], dZ, Yd);
+            Mat dpsi(Np, Np);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dpsi(i, j) = Yd(i, j);
+            gemm(-(sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), sinv[n], psi, 1.0, dpsi);
+            dstore(n, dpsi);
+        }
     }
     auto get_psi = [&](int n) -> const Mat& {
         if (n >= asym_off && n <= N) return psi_asym[n - asym_off];
         if (n >= 0 && n <= n_main_hi) return psi_keep[n];
         throw std::runtime_error("get_psi outside kept ranges");
+    };
+    auto get_dpsi = [&](int n) -> const Mat& {
+        if (n >= asym_off && n <= N) return dpsi_asym[n - asym_off];
+        if (n >= 0 && n <= n_main_hi) return dpsi_keep[n];
+        throw std::runtime_error("get_dpsi outside kept ranges");
     };
     const double t4 = now_s(); ts[4] = t4 - t3;
     // ---- AsymptoticAmplitudes::extract (AsymptoticAmplitudes.cpp:64-359) ----
@@ -1131,6 +1296,56 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             if (closed[lidx[mu]]) continue;
             const double jc = jt[lidx[mu]][i], yc = yt[lidx[mu]][i];
             for (int j = 0; j < Np; ++j) max_resid = std::max(max_resid, std::abs(jc * Af(mu, j) + yc * Bf(mu, j) - ps(mu, j)));
+        }
+    }
+    // tangent of the fit: the normal matrices and the tables move with dk,
+    // YJ / YY with dpsi and the tables
+    Mat dAf, dBf;
+    if (tan) {
+        std::vector<std::vector<double>> djt(nl, std::vector<double>(n_fit)), dyt(nl, std::vector<double>(n_fit));
+        for (int t = 0; t < nl; ++t)
+            for (int i = 0; i < n_fit; ++i) {
+                const double r = h + (n_fs + i) * h;
+                djt[t][i] = riccati_jp(uls[t], k * r) * r * dk;
+                dyt[t][i] = riccati_yp(uls[t], k * r) * r * dk;
+            }
+        std::vector<std::array<double, 4>> dMinv(nl);
+        for (int t = 0; t < nl; ++t) {
+            if (closed[t]) { dMinv[t] = {0, 0, 0, 0}; continue; }
+            double d00 = 0, d01 = 0, d11 = 0;
+            for (int i = 0; i < n_fit; ++i) {
+                d00 += 2.0 * jt[t][i] * djt[t][i];
+                d01 += djt[t][i] * yt[t][i] + jt[t][i] * dyt[t][i];
+                d11 += 2.0 * yt[t][i] * dyt[t][i];
+            }
+            // dMinv = -Minv dM Minv (symmetric 2x2; storage (00,10,01,11))
+            const auto& M = Minv[t];
+            const double a = M[0], b = M[1], c = M[3];
+            const double x00 = -(a * (d00 * a + d01 * b) + b * (d01 * a + d11 * b));
+            const double x01 = -(a * (d00 * b + d01 * c) + b * (d01 * b + d11 * c));
+            const double x11 = -(b * (d00 * b + d01 * c) + c * (d01 * b + d11 * c));
+            dMinv[t] = {x00, x01, x01, x11};
+        }
+        Mat dYJ(Np, Np), dYY(Np, Np);
+        for (int i = 0; i < n_fit; ++i) {
+            const Mat& ps = get_psi(n_fs + i);
+            const Mat& dps = get_dpsi(n_fs + i);
+            for (int j = 0; j < Np; ++j) for (int mu = 0; mu < Np; ++mu) {
+                const int t = lidx[mu];
+                dYJ(mu, j) += djt[t][i] * ps(mu, j) + jt[t][i] * dps(mu, j);
+                dYY(mu, j) += dyt[t][i] * ps(mu, j) + yt[t][i] * dps(mu, j);
+            }
+        }
+        dAf.resize(Np, Np); dBf.resize(Np, Np);
+        for (int mu = 0; mu < Np; ++mu) {
+            const int t = lidx[mu];
+            if (closed[t]) continue;
+            const auto& M = Minv[t];
+            const auto& dM = dMinv[t];
+            for (int j = 0; j < Np; ++j) {
+                dAf(mu, j) = dM[0] * YJ(mu, j) + M[0] * dYJ(mu, j) + dM[2] * YY(mu, j) + M[2] * dYY(mu, j);
+                dBf(mu, j) = dM[1] * YJ(mu, j) + M[1] * dYJ(mu, j) + dM[3] * YY(mu, j) + M[3] * dYY(mu, j);
+            }
         }
     }
     double AmI = 0.0;
@@ -1220,6 +1435,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     // integrands
     std::vector<Mat> integ(6, Mat(n_pts, Np));           // [slot](row, beta)
     std::vector<Mat> integ_b(n_occ, Mat(n_pts, Np));     // [alpha](row, beta)
+    std::vector<Mat> dinteg(tan ? 6 : 0, Mat(n_pts, Np)), dinteg_b(tan ? n_occ : 0, Mat(n_pts, Np));
     for (int ir = n_hi - 1; ir >= n_lo; --ir) {
         const int row = ir - n_lo;
         const double r = h + ir * h;
@@ -1238,9 +1454,28 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
                 for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * ps(mu, b);
                 integ_b[a](row, b) = s;
             }
+        if (tan) {
+            const Mat& dps = get_dpsi(ir);
+            for (int slot = 0; slot < 6; ++slot) {
+                const double rf = (slot < 3) ? r : 1.0;
+                for (int b = 0; b < Np; ++b) {
+                    double s = 0.0;
+                    for (int mu = 0; mu < Np; ++mu) s += Xi[slot](row, mu) * dps(mu, b);
+                    dinteg[slot](row, b) = rf * s;
+                }
+            }
+            for (int a = 0; a < n_occ; ++a)
+                for (int b = 0; b < Np; ++b) {
+                    double s = 0.0;
+                    for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * dps(mu, b);
+                    dinteg_b[a](row, b) = s;
+                }
+        }
     }
-    Mat bov(Np, n_occ);
+    Mat bov(Np, n_occ), dbov(tan ? Np : 0, tan ? n_occ : 0);
     for (int a = 0; a < n_occ; ++a) for (int b = 0; b < Np; ++b) bov(b, a) = simpson(integ_b[a].col(b), n_pts, h);
+    if (tan)
+        for (int a = 0; a < n_occ; ++a) for (int b = 0; b < Np; ++b) dbov(b, a) = simpson(dinteg_b[a].col(b), n_pts, h);
     CMat M(Np, Np);
     {
         CMat AmiB(Np, Np);
@@ -1250,6 +1485,27 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     }
     CMat o(Np, n_occ);
     for (int a = 0; a < n_occ; ++a) for (int p = 0; p < Np; ++p) { const cd b = bov(p, a); for (int i = 0; i < Np; ++i) o(i, a) += M(i, p) * b; }
+    // tangent: M = (C^{-1})^dagger with C = A - iB, dM = -M dC^dagger M, so
+    // d(M x) = M (dx - dC^dagger (M x)) for any E-dependent x
+    auto mtan = [&](const std::vector<cd>& Mx, const std::vector<double>& dx) {
+        std::vector<cd> v(Np);
+        for (int i = 0; i < Np; ++i) {
+            cd s(dx[i], 0.0);
+            for (int p = 0; p < Np; ++p) s -= cd(dAf(p, i), dBf(p, i)) * Mx[p];   // (dC^dagger)(i,p) = conj(dC(p,i))
+            v[i] = s;
+        }
+        std::vector<cd> r(Np, cd(0, 0));
+        for (int p = 0; p < Np; ++p) for (int i = 0; i < Np; ++i) r[i] += M(i, p) * v[p];
+        return r;
+    };
+    CMat dO(tan ? Np : 0, tan ? n_occ : 0);
+    if (tan)
+        for (int a = 0; a < n_occ; ++a) {
+            std::vector<cd> oa(Np); std::vector<double> dba(Np);
+            for (int i = 0; i < Np; ++i) { oa[i] = o(i, a); dba[i] = dbov(i, a); }
+            const std::vector<cd> r = mtan(oa, dba);
+            for (int i = 0; i < Np; ++i) dO(i, a) = r[i];
+        }
     for (int slot = 0; slot < 6; ++slot) {
         std::vector<double> draw(Np);
         for (int b = 0; b < Np; ++b) draw[b] = simpson(integ[slot].col(b), n_pts, h);
@@ -1273,6 +1529,18 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             for (int a = 0; a < n_occ; ++a) sub += 2.0 * o(mu, a) * dcorr[a];
             D[mu] = Draw[mu] - sub;
         }
+        if (tan) {
+            std::vector<double> ddraw(Np);
+            for (int b = 0; b < Np; ++b) ddraw[b] = simpson(dinteg[slot].col(b), n_pts, h);
+            const std::vector<cd> dDraw = mtan(Draw, ddraw);
+            for (int mu = 0; mu < Np; ++mu) {
+                cd sub = 0.0;
+                for (int a = 0; a < n_occ; ++a) sub += 2.0 * dO(mu, a) * dcorr[a];
+                const cd dDm = dDraw[mu] - sub;
+                if (out->dD) { out->dD[2 * (slot * Np + mu)] = dDm.real(); out->dD[2 * (slot * Np + mu) + 1] = dDm.imag(); }
+            }
+            if (out->dd_raw) for (int b = 0; b < Np; ++b) out->dd_raw[slot * Np + b] = ddraw[b];
+        }
         if (out->d_raw) for (int b = 0; b < Np; ++b) out->d_raw[slot * Np + b] = draw[b];
         if (out->D_raw) for (int b = 0; b < Np; ++b) { out->D_raw[2 * (slot * Np + b)] = Draw[b].real(); out->D_raw[2 * (slot * Np + b) + 1] = Draw[b].imag(); }
         if (out->D) for (int b = 0; b < Np; ++b) { out->D[2 * (slot * Np + b)] = D[b].real(); out->D[2 * (slot * Np + b) + 1] = D[b].imag(); }
@@ -1282,6 +1550,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     // ---- outputs ----
     auto put = [](double* dst, const Mat& M) { if (dst) std::memcpy(dst, M.a.data(), M.a.size() * sizeof(double)); };
     put(out->K_kme, Kk); put(out->A, Af); put(out->B, Bf); put(out->K_fit, Kfit); put(out->b_overlap, bov);
+    if (tan) { put(out->dK, dKk); put(out->dA, dAf); put(out->dB, dBf); put(out->db_overlap, dbov); }
     if (out->diag) {
         double* d = out->diag;
         d[0] = max_resid / std::max(max_psi, 1e-30); d[1] = max_resid; d[2] = Ksym; d[3] = AmI;
@@ -1291,6 +1560,10 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     for (int i = 0; i < out->n_dump_sinv; ++i) put(out->dump_sinv + static_cast<size_t>(i) * Np * Np, sinv[out->dump_sinv_n[i]]);
     for (int i = 0; i < out->n_dump_rinv; ++i) put(out->dump_rinv + static_cast<size_t>(i) * Nt * Nt, rinv[out->dump_rinv_n[i]]);
     for (int i = 0; i < out->n_dump_psi; ++i) put(out->dump_psi + static_cast<size_t>(i) * Np * Np, get_psi(out->dump_psi_n[i]));
+    if (tan && out->dump_drinv)
+        for (int i = 0; i < out->n_dump_rinv; ++i) put(out->dump_drinv + static_cast<size_t>(i) * Nt * Nt, drinv[out->dump_rinv_n[i]]);
+    if (tan && out->dump_dpsi)
+        for (int i = 0; i < out->n_dump_psi; ++i) put(out->dump_dpsi + static_cast<size_t>(i) * Np * Np, get_dpsi(out->dump_psi_n[i]));
     if (out->t_stage) for (int i = 0; i < 8; ++i) out->t_stage[i] = ts[i];
 }
 

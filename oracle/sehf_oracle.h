@@ -59,6 +59,19 @@ typedef struct {
     int  n_dump_rinv; const int* dump_rinv_n; double* dump_rinv;
     int  n_dump_psi;  const int* dump_psi_n;  double* dump_psi;
     double* t_stage;      /* [8] seconds: setup, sinv, frp, kme, bp, fit, dipole, total */
+    /* tangent along (E, alpha): derivative of every output below with respect
+     * to E times tan_E plus with respect to the polarisation strength alpha
+     * times tan_alpha (computed only when tangent != 0) */
+    int     tangent;
+    double  tan_E, tan_alpha;
+    double* dK;           /* [N*N]  of K_kme                                        */
+    double* dA;           /* [N*N]                                                  */
+    double* dB;           /* [N*N]                                                  */
+    double* db_overlap;   /* [N*n_occ]                                              */
+    double* dd_raw;       /* [6*N]                                                  */
+    double* dD;           /* [6*N*2] complex interleaved                            */
+    double* dump_dpsi;    /* [n_dump_psi*N*N] dpsi at dump_psi_n                    */
+    double* dump_drinv;   /* [n_dump_rinv*N_tot^2] dRinv at dump_rinv_n             */
 } sehf_oracle_out;
 
 /* Solve one energy point E = k^2/2.  Returns 0 on success, else writes a
