@@ -62,6 +62,50 @@ def cpu_baseline(p, ks, n_steps):
                       f"{energy_s:.0f} s per energy (extrapolated)"}
 
 
+def side_measurements(args, p, ks_all, B):
+    """Short-grid ratios (same channel sizes, r_max = args.side_rmax; per-step
+    costs do not depend on the grid length):
+    * batched vs the plain per-ik path on this configuration: B energies in
+      one batch against one energy alone (batch 1);
+    * the forward-mode derivative's cost on the l_cont = 20 polarisation
+      configuration: the same B energies with and without the tangent."""
+    import torch
+    from sehf import scan as sc, synth
+    out = {"r_max": args.side_rmax}
+    q = synth.h2o_like(p.l_cont, p.dr, args.side_rmax, p.n_occ, seed=1, allow_closed=True)
+    S = sc.Scan(q)
+    ks = ks_all[(np.arange(B) * 37) % len(ks_all)]
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        f()
+        torch.cuda.synchronize()
+        return time.time() - t0
+
+    S.run(ks[:1], batch=1)                                    # warm the per-ik buffers
+    t_b = timed(lambda: S.run(ks, batch=B, groups=1))
+    t_1 = timed(lambda: S.run(ks[:1], batch=1, groups=1))
+    out["per_ik"] = {"workload": f"{args.config} channels, Nr={q.Nr}", "batched_energies_per_s": B / t_b,
+                     "per_ik_energies_per_s": 1.0 / t_1, "batched_over_per_ik": (B / t_b) * t_1}
+    print(f"[bench] side: batched {B / t_b:.3f} vs per-ik {1.0 / t_1:.3f} energies/s on Nr={q.Nr}",
+          file=sys.stderr, flush=True)
+    del S
+    pc, kc = synth.config("h2o_l20_pol")
+    q = synth.h2o_like(pc.l_cont, pc.dr, args.side_rmax, pc.n_occ, seed=1, pol=(pc.pol_alpha, pc.pol_rc),
+                       allow_closed=True)
+    S = sc.Scan(q)
+    ks = kc[(np.arange(B) * 37) % len(kc)]
+    S.run(ks, batch=B, tangent=(1.0, 0.0))                   # allocate the tangent buffers
+    t_f = timed(lambda: S.run(ks, batch=B))
+    t_t = timed(lambda: S.run(ks, batch=B, tangent=(1.0, 0.0)))
+    out["tangent"] = {"workload": f"h2o_l20_pol channels (N_tot={q.n_tot}), Nr={q.Nr}, batch {B}",
+                      "forward_energies_per_s": B / t_f, "with_tangent_energies_per_s": B / t_t,
+                      "tangent_over_forward_cost": t_t / t_f}
+    print(f"[bench] side: tangent/forward cost {t_t / t_f:.2f} on Nr={q.Nr}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +120,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-every", type=int, default=97)
     ap.add_argument("--per-ik", action="store_true",
-                    help="also time the plain per-ik path (batch 1, one group) on one energy")
+                    help="also time the plain per-ik path (batch 1, one group) on one energy of the full grid")
+    ap.add_argument("--tangent", action="store_true",
+                    help="time the batched scan with its forward-mode derivative along E (batched + adjoint)")
+    ap.add_argument("--side-rmax", type=float, default=5.0,
+                    help="radial extent of the short-grid side measurements (per-ik vs batched on this config; "
+                         "tangent vs plain on the l_cont=20 polarisation config); 0 skips them")
     ap.add_argument("--rmax", type=float, default=0.0,
                     help="profiling aid: shorten the radial grid (same channel sizes); not the metric workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -128,15 +177,16 @@ def main():
         del src, dst
         print(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written",
               file=sys.stderr, flush=True)
+    tan = (1.0, 0.0) if args.tangent else None
     for w in range(args.warmup):
-        S.run(batch_k(w), batch=B, seg_len=args.seg_len, groups=G)
+        S.run(batch_k(w), batch=B, seg_len=args.seg_len, groups=G, tangent=tan)
         print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     L.sehf_prof_begin(args.prof_every)
     t0 = time.time()
     outs = []
     for i in range(args.steps):
-        outs.append(S.run(batch_k(args.warmup + i), batch=B, seg_len=args.seg_len, groups=G))
+        outs.append(S.run(batch_k(args.warmup + i), batch=B, seg_len=args.seg_len, groups=G, tangent=tan))
         print(f"[bench] timed step {i} done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
@@ -158,7 +208,7 @@ def main():
     stages = S.stats()
     for o in outs:
         for r in o:
-            if not np.all(np.isfinite(r["D"])):
+            if not np.all(np.isfinite(r["D"])) or (args.tangent and not np.all(np.isfinite(r["dD"]))):
                 raise SystemExit("non-finite dipole in bench output")
 
     line = {
@@ -168,8 +218,10 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), "
                                f"Nr={p.Nr}, dr={p.dr}, k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
-                               f"batch {B} energies per step, closed channels allowed",
-                   "batch_energies": B, "groups": G, "energies_per_step": E_step,
+                               f"batch {B} energies per step, closed channels allowed"
+                               + (", with the forward-mode derivative along E (batched + adjoint)" if args.tangent
+                                  else ""),
+                   "batch_energies": B, "groups": G, "energies_per_step": E_step, "tangent": bool(args.tangent),
                    "seg_len": args.seg_len or "ceil(sqrt(Nr))"},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
@@ -190,6 +242,8 @@ def main():
         t1 = time.time() - t0
         line["per_ik"] = {"value": 1.0 / t1, "unit": "energy points/s", "seconds": t1,
                           "batched_over_per_ik": value * t1}
+    if args.side_rmax > 0 and args.rmax == 0:
+        line["side"] = side_measurements(args, p, ks_all, B)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, batch_k(0), args.cpu_steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

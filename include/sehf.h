@@ -142,6 +142,30 @@ int  sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out);
 /* info[8] = N_psi, N_f, N_total, n_transition, n_start, dipole points, Nr, l_exch */
 int  sehf_scan_info(const sehf_scan* s, int* info);
 int  sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out);
+
+/* Derivatives of the per-energy outputs along a direction t_E d/dE +
+ * t_alpha d/dalpha (E = k^2/2 in Hartree, alpha the polarisation strength
+ * pol_alpha of sehf_problem).  The reference has no derivative: it obtains
+ * dd/dE for the Wigner delay by finite differences of d over the k grid
+ * (postprocessing/cross_section_delay.py:227-263), and has no d/dalpha.  The
+ * layouts are those of sehf_scan_out; any pointer may be NULL. */
+typedef struct {
+    double* dK;          /* [nk][N*N]  of K                                         */
+    double* dA;          /* [nk][N*N]  of A                                         */
+    double* dB;          /* [nk][N*N]  of B                                         */
+    double* db_overlap;  /* [nk][n_occ][N]                                          */
+    double* dd_raw;      /* [nk][6][N]                                              */
+    double* dD;          /* [nk][6][N] complex interleaved: of the reduced d_lm     */
+} sehf_scan_tangent_out;
+
+/* sehf_scan_run plus the forward-mode derivative (tangent) of the whole stage
+ * chain: the ratio matrices' derivatives are propagated with the forward
+ * recursion and checkpointed with them, the back-propagation carries
+ * dZ = d(Rinv Z) beside Z, and the K match, fit and dipole normalisation are
+ * differentiated in closed form.  dir[2*i], dir[2*i+1] = (t_E, t_alpha) of
+ * energy i; the same k may appear twice with two directions. */
+int  sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len,
+                           sehf_scan_out* out, sehf_scan_tangent_out* tout);
 /* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
  * fit + dipole, per-batch host tables. */
 int  sehf_scan_stats(const sehf_scan* s, double* t);

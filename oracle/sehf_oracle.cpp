@@ -194,8 +194,11 @@ static Mat inverse_gj(const Mat& A) {
     for (int k = n - 1; k >= 0; --k) if (piv[k] != k) for (int i = 0; i < n; ++i) std::swap(a(i, k), a(i, piv[k]));
     return a;
 }
-static int g_inverse_mode = 0;   // 0: LU (the reference), 1: Gauss-Jordan (noise-floor probe)
-static Mat inverse_general(const Mat& A) { return g_inverse_mode == 1 ? inverse_gj(A) : inverse_pp(A); }
+// Rounding-order probes (noise-floor measurements only): bit 0 selects
+// Gauss-Jordan in place of LU for the dense inverses, bit 1 accumulates the
+// asymptotic-fit sums over the window from its outer end inwards.
+static int g_inverse_mode = 0;
+static Mat inverse_general(const Mat& A) { return (g_inverse_mode & 1) ? inverse_gj(A) : inverse_pp(A); }
 // inverse_symmetric_indefinite, no-MKL branch (LapackInverse.hpp:174-176).
 static Mat inverse_symmetric_indefinite(const Mat& A) { Mat X = inverse_general(A); symmetrize(X); return X; }
 
@@ -1272,7 +1275,8 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     }
     double max_resid = 0.0, max_psi = 0.0;
     Mat YJ(Np, Np), YY(Np, Np);
-    for (int i = 0; i < n_fit; ++i) {
+    for (int ii = 0; ii < n_fit; ++ii) {
+        const int i = (g_inverse_mode & 2) ? n_fit - 1 - ii : ii;
         const Mat& ps = get_psi(n_fs + i);
         max_psi = std::max(max_psi, max_abs(ps));
         for (int j = 0; j < Np; ++j) for (int mu = 0; mu < Np; ++mu) {
@@ -1327,7 +1331,8 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             dMinv[t] = {x00, x01, x01, x11};
         }
         Mat dYJ(Np, Np), dYY(Np, Np);
-        for (int i = 0; i < n_fit; ++i) {
+        for (int ii = 0; ii < n_fit; ++ii) {
+            const int i = (g_inverse_mode & 2) ? n_fit - 1 - ii : ii;
             const Mat& ps = get_psi(n_fs + i);
             const Mat& dps = get_dpsi(n_fs + i);
             for (int j = 0; j < Np; ++j) for (int mu = 0; mu < Np; ++mu) {

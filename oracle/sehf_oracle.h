@@ -92,11 +92,14 @@ int sehf_oracle_time_steps_multi(const sehf_oracle_problem* p, const double* ks,
 
 const char* sehf_oracle_last_error(void);
 
-/* Dense-inverse elimination order: 0 = partial-pivot LU as the reference
- * (default); 1 = Gauss-Jordan with partial pivoting.  Mode 1 is a second,
- * equally valid rounding order, used by tests only to measure the rounding
- * noise floor of the reference algorithm.  Process-wide; not thread-safe
- * against concurrent solves. */
+/* Rounding-order probes, bit flags (0 = the reference's order, default):
+ * bit 0: Gauss-Jordan with partial pivoting in place of partial-pivot LU for
+ *        the dense inverses;
+ * bit 1: the asymptotic-fit window sums accumulated from the outer end
+ *        inwards (the order of a back-propagation sweep).
+ * Each is a second, equally valid rounding order of the reference algorithm,
+ * used by tests only to measure its rounding noise floor.  Process-wide; not
+ * thread-safe against concurrent solves. */
 void sehf_oracle_set_inverse_mode(int mode);
 
 /* ---- primitives exposed for golden-vector tests ---- */

@@ -288,6 +288,15 @@ double ric_y(int l, double x) {
     if (x < 1e-300) throw std::runtime_error("Riccati-Neumann evaluated at x ~ 0");
     return x * sph_y(l, x);
 }
+double ric_jp(int l, double x) {
+    if (l == 0) return std::cos(x);
+    if (x < 1e-300) return 0.0;
+    return ric_j(l - 1, x) - static_cast<double>(l) / x * ric_j(l, x);
+}
+double ric_yp(int l, double x) {
+    if (l == 0) return std::sin(x);
+    return ric_y(l - 1, x) - static_cast<double>(l) / x * ric_y(l, x);
+}
 
 // ---------------------------------------------------------------------------
 // Smallest eigenvalue: Householder reduction to tridiagonal form, then

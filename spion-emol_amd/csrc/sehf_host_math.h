@@ -47,6 +47,10 @@ double real_ylm(int l, int m, double theta, double phi);
 // Riccati-Bessel  x j_l(x)  and Riccati-Neumann  x y_l(x)  (y_0 = -cos x).
 double ric_j(int l, double x);
 double ric_y(int l, double x);
+// Their x-derivatives from the recurrence u_l' = u_{l-1} - (l/x) u_l
+// (u_0' = cos x for j, sin x for y).
+double ric_jp(int l, double x);
+double ric_yp(int l, double x);
 
 // Smallest eigenvalue of a symmetric n x n matrix given column-major
 // (lower triangle read).

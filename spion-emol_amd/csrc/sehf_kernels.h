@@ -12,9 +12,10 @@
 
 namespace sehf {
 
-// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]   (column-major)
-// Columns of C in [zc0, zc1) take beta = 0 (pivot-block rows of the
-// Gauss-Jordan update, see gj_inverse).  A stride of 0 shares an operand
+// C[b] = alpha * alpha_b[b] * op(A[b]) * op(B[b]) + beta * C[b]   (column-major)
+// alpha_b (device, one factor per batch entry) is optional; without it the
+// factor is 1.  Columns of C in [zc0, zc1) take beta = 0 (pivot-block rows of
+// the Gauss-Jordan update, see gj_inverse).  A stride of 0 shares an operand
 // across the batch.
 struct Gemm {
     int M = 0, N = 0, K = 0;
@@ -22,6 +23,7 @@ struct Gemm {
     const double* B = nullptr; int64_t ldb = 0, sB = 0; int tB = 0;
     double* C = nullptr; int64_t ldc = 0, sC = 0;
     double alpha = 1.0, beta = 0.0;
+    const double* alpha_b = nullptr;
     int zc0 = 0, zc1 = 0;
     int batch = 1;
 };

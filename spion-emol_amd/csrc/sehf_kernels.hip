@@ -39,6 +39,7 @@ struct GemmK {
     const double* B; int64_t ldb, sB; int tB;
     double* C; int64_t ldc, sC;
     double alpha, beta;
+    const double* alpha_b;
     int zc0, zc1;
 };
 
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
                     acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[bj], av[bi], acc[bi][bj], 0, 0, 0);
         }
     }
+    const double al = p.alpha_b ? p.alpha * p.alpha_b[b] : p.alpha;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
                 if (row < p.M && col < p.N) {
                     double* c = C + row + (int64_t)col * p.ldc;
                     const double be = (col >= p.zc0 && col < p.zc1) ? 0.0 : p.beta;
-                    double val = p.alpha * acc[bi][bj][v];
+                    double val = al * acc[bi][bj][v];
                     if (be != 0.0) val += be * (*c);
                     *c = val;
                 }
@@ -225,7 +227,7 @@ struct ProfScope {
 void gemm(const Gemm& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
     GemmK p{g.M, g.N, g.K, g.A, g.lda, g.sA, g.tA, g.B, g.ldb, g.sB, g.tB, g.C, g.ldc, g.sC,
-            g.alpha, g.beta, g.zc0, g.zc1};
+            g.alpha, g.beta, g.alpha_b, g.zc0, g.zc1};
     // sampled timing buckets by depth: 2 (K < 64), 3 (64 <= K < 256), 4 (K >= 256)
     ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, g.K < 64 ? 2 : (g.K < 256 ? 3 : 4));
     dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);

@@ -49,6 +49,11 @@ struct Batch {
     DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
     GjBuffers gj;             // inversions on the main stream
     GjBuffers gj2;            // Schur inversions on the side stream
+    // tangent along (E, alpha), allocated by the first tangent run
+    bool has_tan = false;
+    DevArray<double> dRp, dRc, dckpt, dsegR, Gt, dZ, dZt, P1, dPsi, dacc, dYJ, dYY, dAm, dBm, dKd, dP, Q1, Q2;
+    DevArray<double> vre, vim, dDRre, dDRim, dore, doim, dDre, dDim;
+    DevArray<double> wF, wP, wN, danl, dtab, djy, dJc, dYc, dcoef;
 };
 
 // One energy group: its own stream and batch buffers.  Groups run
@@ -267,10 +272,10 @@ void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_ou
     if (meS < thr) { dS_out = thr - meS + Setup::shift_margin; ++shS; }
 }
 
-void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit) {
+void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, bool tan) {
     Batch& b = g.bt;
     if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_seg, 1)) * B * s->Nt * s->Nt &&
-        b.win.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np * s->Np)
+        b.win.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np * s->Np && (!tan || b.has_tan))
         return;
     const size_t Np = s->Np, Nf = s->Nf, Nt = s->Nt, ns = s->nslot;
     const size_t tt = Nt * Nt, pp = Np * Np, pf = Np * std::max<size_t>(Nf, 1), ff = std::max<size_t>(Nf, 1) * std::max<size_t>(Nf, 1);
@@ -298,6 +303,24 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit) {
     b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
     b.gj2.ensure(static_cast<int>(Np), B);
     b.SinvR.alloc(2 * pp * B);
+    if (tan) {
+        const size_t no = s->S->n_occ, nx = std::max<size_t>(6, no);
+        b.has_tan = true;
+        b.dRp.alloc(tt * B); b.dRc.alloc(tt * B);
+        b.dckpt.alloc(tt * B * std::max(n_seg, 1)); b.dsegR.alloc(tt * B * L);
+        b.Gt.alloc(Np * Nt * B); b.dZ.alloc(Nt * Np * B); b.dZt.alloc(Nt * Np * B);
+        b.P1.alloc(pp * B); b.dPsi.alloc(pp * B); b.dacc.alloc(ns * Np * B);
+        b.dYJ.alloc(pp * B); b.dYY.alloc(pp * B); b.dAm.alloc(pp * B); b.dBm.alloc(pp * B); b.dKd.alloc(pp * B);
+        b.dP.alloc(pp * B); b.Q1.alloc(pp * B); b.Q2.alloc(pp * B);
+        b.vre.alloc(Np * nx * B); b.vim.alloc(Np * nx * B);
+        b.dDRre.alloc(6 * Np * B); b.dDRim.alloc(6 * Np * B); b.dore.alloc(Np * no * B); b.doim.alloc(Np * no * B);
+        b.dDre.alloc(6 * Np * B); b.dDim.alloc(6 * Np * B);
+        b.wF.alloc(static_cast<size_t>(s->Nr) * B); b.wP.alloc(static_cast<size_t>(s->Nr) * B); b.wN.alloc(B);
+        b.danl.alloc(static_cast<size_t>(s->n_start + 1) * B * Nt);
+        b.dtab.alloc(4 * Np * B); b.djy.alloc(2 * Np * B);
+        b.dJc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.dYc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
+        b.dcoef.alloc(4 * Np * B);
+    }
     g.ensure_seg_events(L);
 }
 
@@ -309,7 +332,11 @@ double wall() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_scan_out* out, int out_off) {
+// One batch of energies through the whole path.  With `dir` (per energy the
+// direction (t_E, t_alpha)) the forward-mode derivative of every output along
+// t_E d/dE + t_alpha d/dalpha is propagated alongside (tout receives it).
+void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int nb, int L, sehf_scan_out* out,
+               sehf_scan_tangent_out* tout, int out_off) {
     const double tw0 = wall();
     Setup& S = *s->S;
     hipStream_t st = g.st;
@@ -318,6 +345,11 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     const size_t tt = static_cast<size_t>(Nt) * Nt, pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * std::max(Nf, 1);
     const int n_seg = (N + L - 1) / L;
     const double h = s->h;
+    const bool tan = dir != nullptr;
+    // -Sinv (B D^{-1}) of the current step: inside Gt = [Sinv | -Sinv B D^{-1}]
+    // (= (W^{-1})[:, psi]^T) for tangent runs, else its own buffer
+    double* const Xd = tan ? b.Gt.p + pp : b.X.p;
+    const int64_t sXd = tan ? static_cast<int64_t>(Np) * Nt : static_cast<int64_t>(pf);
     // ---------------- host tables for this batch ----------------
     std::vector<double> E(nb), cE(nb);
     for (int e = 0; e < nb; ++e) { E[e] = 0.5 * ks[e] * ks[e]; cE[e] = s->c6 * E[e]; }
@@ -375,6 +407,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     if (n_fe - n_fs + 1 < 10) throw std::runtime_error("asymptotic fit window is too small");
     const int n_fit = n_fe - n_fs + 1;
     std::vector<double> Jc(static_cast<size_t>(n_fit) * Np * nb), Yc(Jc.size()), coef(static_cast<size_t>(5) * Np * nb);
+    std::vector<double> dJc(tan ? Jc.size() : 0), dYc(dJc.size()), dcoef(tan ? static_cast<size_t>(4) * Np * nb : 0);
     std::vector<int> n_closed(nb, 0);
     {
         int lmax = 0;
@@ -417,10 +450,109 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
                     Yc[(static_cast<size_t>(e) * n_fit + i) * Np + mu] = yl[static_cast<size_t>(l) * n_fit + i];
                 }
             }
+            if (!tan) continue;
+            // tangent of the fit tables: dj = j'(kr) r dk, and of the 2x2
+            // inverse normal matrices, dMinv = -Minv dM Minv
+            const double dk = dir[2 * e] / k;
+            std::vector<double> djl(jl.size()), dyl(jl.size()), dm(4 * (lmax + 1), 0.0);
+            for (int l = 0; l <= lmax; ++l) {
+                double d00 = 0, d01 = 0, d11 = 0;
+                for (int i = 0; i < n_fit; ++i) {
+                    const size_t q = static_cast<size_t>(l) * n_fit + i;
+                    const double r = h + (n_fs + i) * h;
+                    djl[q] = ric_jp(l, k * r) * r * dk;
+                    dyl[q] = ric_yp(l, k * r) * r * dk;
+                    d00 += 2.0 * jl[q] * djl[q];
+                    d01 += djl[q] * yl[q] + jl[q] * dyl[q];
+                    d11 += 2.0 * yl[q] * dyl[q];
+                }
+                if (m[5 * l + 4] != 0.0) continue;   // closed: A = I, B = 0 do not move
+                const double a = m[5 * l + 0], bq = m[5 * l + 1], c = m[5 * l + 3];
+                const double x00 = -(a * (d00 * a + d01 * bq) + bq * (d01 * a + d11 * bq));
+                const double x01 = -(a * (d00 * bq + d01 * c) + bq * (d01 * bq + d11 * c));
+                const double x11 = -(bq * (d00 * bq + d01 * c) + c * (d01 * bq + d11 * c));
+                dm[4 * l + 0] = x00; dm[4 * l + 1] = x01; dm[4 * l + 2] = x01; dm[4 * l + 3] = x11;
+            }
+            double* dcf = dcoef.data() + static_cast<size_t>(e) * 4 * Np;
+            for (int mu = 0; mu < Np; ++mu) {
+                const int l = S.l_mu[mu];
+                for (int q = 0; q < 4; ++q) dcf[q * Np + mu] = dm[4 * l + q];
+                for (int i = 0; i < n_fit; ++i) {
+                    dJc[(static_cast<size_t>(e) * n_fit + i) * Np + mu] = djl[static_cast<size_t>(l) * n_fit + i];
+                    dYc[(static_cast<size_t>(e) * n_fit + i) * Np + mu] = dyl[static_cast<size_t>(l) * n_fit + i];
+                }
+            }
         }
     }
     up(b.dA.p, dA, st); up(b.dS.p, dS, st); up(b.cE.p, cE, st); up(b.anl.p, anl, st);
     up(b.tab.p, tab, st); up(b.jy.p, jy, st); up(b.Jc.p, Jc, st); up(b.Yc.p, Yc, st); up(b.coef.p, coef, st);
+    // tangent tables (kept alive until the stream is synchronised below)
+    std::vector<double> wF, wP, wN, danl, dtab, djy;
+    if (tan) {
+        // Tangent tables.  E and alpha enter A_n = I + h^2/6 (E - V_n) only as
+        // h^2/6 w_n I with w_n = t_E - t_alpha dV_pol/dalpha(n); a stability
+        // shift at n rebuilds the matrix with a fixed smallest eigenvalue and
+        // removes that dependence (sig = 0).  Per energy and n:
+        //   wF = 12 sig h^2/6 w_n  (the derivative of 12 W^{-1}: -12 sig h^2/6 w G G^T
+        //                           enters U, and d(U^{-1}) = -U^{-1} dU U^{-1})
+        //   wP = -sig h^2/6 w_n    (dW^{-1} applied to Z: -sig h^2/6 w G psi)
+        wF.assign(static_cast<size_t>(Nr) * nb, 0.0); wP.assign(wF.size(), 0.0); wN.assign(nb, 0.0);
+        for (int n = 0; n < Nr; ++n) {
+            const double pu = S.polarization_per_alpha(n);
+            for (int e = 0; e < nb; ++e) {
+                const size_t q = static_cast<size_t>(n) * nb + e;
+                const bool shifted = n < s->nstab && (dA[q] != 0.0 || dS[q] != 0.0);
+                const double w = dir[2 * e] - dir[2 * e + 1] * pu, sg = shifted ? 0.0 : 1.0;
+                wF[q] = 12.0 * sg * s->c6 * w;
+                wP[q] = -sg * s->c6 * w;
+                if (n == N) wN[e] = s->c6 * w;
+            }
+        }
+        // analytic start (n < n_start): derivative of the diagonal ratio
+        // M_n j(k r_n) / (M_{n+1} j(k r_{n+1})); the f block is constant
+        danl.assign(static_cast<size_t>(ns0) * nb * Nt, 0.0);
+        for (int n = 1; n < ns0; ++n) {
+            const double pn = S.polarization_per_alpha(n), pn1 = S.polarization_per_alpha(n + 1);
+            for (int e = 0; e < nb; ++e) {
+                double* d = danl.data() + (static_cast<size_t>(n) * nb + e) * Nt;
+                const double k = ks[e], dk = dir[2 * e] / k, h2_6 = h * h / 6.0;
+                const double wn = dir[2 * e] - dir[2 * e + 1] * pn, wn1 = dir[2 * e] - dir[2 * e + 1] * pn1;
+                const double rn = h + n * h, rn1 = h + (n + 1) * h;
+                for (int mu = 0; mu < Np; ++mu) {
+                    const int l = S.l_mu[mu];
+                    const double Mn = 1.0 + h2_6 * (E[e] - s->vdiag[static_cast<size_t>(n) * Np + mu]);
+                    const double Mn1 = 1.0 + h2_6 * (E[e] - s->vdiag[static_cast<size_t>(n + 1) * Np + mu]);
+                    const double yn = ric_j(l, k * rn), yn1 = ric_j(l, k * rn1);
+                    const double dyn = ric_jp(l, k * rn) * rn * dk, dyn1 = ric_jp(l, k * rn1) * rn1 * dk;
+                    const double a = Mn * yn, bq = Mn1 * yn1;
+                    const double da = h2_6 * wn * yn + Mn * dyn, db = h2_6 * wn1 * yn1 + Mn1 * dyn1;
+                    if (std::fabs(bq) > 1e-300) d[mu] = (da * bq - a * db) / (bq * bq);
+                }
+            }
+        }
+        // K matching and psi_N tables: d(W J) etc. with dW = h^2/12 2 k dk
+        dtab.assign(static_cast<size_t>(4) * Np * nb, 0.0); djy.assign(static_cast<size_t>(2) * Np * nb, 0.0);
+        for (int e = 0; e < nb; ++e) {
+            const double k = ks[e], dk = dir[2 * e] / k, dW = h * h / 12.0 * 2.0 * k * dk;
+            const double* t = tab.data() + static_cast<size_t>(e) * 6 * Np;
+            double* d = dtab.data() + static_cast<size_t>(e) * 4 * Np;
+            double* q = djy.data() + static_cast<size_t>(e) * 2 * Np;
+            for (int mu = 0; mu < Np; ++mu) {
+                const int l = S.l_mu[mu];
+                const double dJin = ric_jp(l, k * r_in) * r_in * dk, dJout = ric_jp(l, k * r_out) * r_out * dk;
+                const double dYin = ric_yp(l, k * r_in) * r_in * dk, dYout = ric_yp(l, k * r_out) * r_out * dk;
+                d[mu] = dW * t[Np + mu] + t[5 * Np + mu] * dJout;
+                d[Np + mu] = dW * t[mu] + t[4 * Np + mu] * dJin;
+                d[2 * Np + mu] = dW * t[2 * Np + mu] + t[4 * Np + mu] * dYin;
+                d[3 * Np + mu] = dW * t[3 * Np + mu] + t[5 * Np + mu] * dYout;
+                q[mu] = ric_jp(l, k * rN) * rN * dk;
+                q[Np + mu] = ric_yp(l, k * rN) * rN * dk;
+            }
+        }
+        up(b.wF.p, wF, st); up(b.wP.p, wP, st); up(b.wN.p, wN, st); up(b.danl.p, danl, st);
+        up(b.dtab.p, dtab, st); up(b.djy.p, djy, st); up(b.dJc.p, dJc, st); up(b.dYc.p, dYc, st);
+        up(b.dcoef.p, dcoef, st);
+    }
     // the side stream reads the per-energy tables (cE, shifts) just uploaded
     hipStream_t st2 = g.st2;
     SEHF_HIP_CHECK(hipEventRecord(g.ev_in, st));
@@ -443,22 +575,48 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout) {
         init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
         if (n < s->n_tr && Nf > 0) {
-            Gemm g;   // X = Sinv (B D^{-1})
+            Gemm g;   // Xn = -Sinv (B D^{-1})
             g.M = Np; g.N = Nf; g.K = Np;
             g.A = Sv; g.lda = Np; g.sA = pp;
             g.B = s->BD.p + static_cast<size_t>(n) * pf; g.ldb = Np; g.sB = 0;
-            g.C = b.X.p; g.ldc = Np; g.sC = pf; g.batch = nb;
+            g.C = Xd; g.ldc = Np; g.sC = sXd; g.alpha = -1.0; g.batch = nb;
             gemm(g, st);
-            add_psi_f_blocks(b.U.p, b.X.p, -12.0, Np, Nf, Nt, nb, st);
-            Gemm h2;  // U_ff += 12 (B D^{-1})^T X
+            add_psi_f_blocks(b.U.p, Xd, sXd, 12.0, Np, Nf, Nt, nb, st);
+            Gemm h2;  // U_ff -= 12 (B D^{-1})^T Xn
             h2.M = Nf; h2.N = Nf; h2.K = Np;
             h2.A = s->BD.p + static_cast<size_t>(n) * pf; h2.lda = Np; h2.sA = 0; h2.tA = 1;
-            h2.B = b.X.p; h2.ldb = Np; h2.sB = pf;
+            h2.B = Xd; h2.ldb = Np; h2.sB = sXd;
             h2.C = b.U.p + Np + static_cast<size_t>(Np) * Nt; h2.ldc = Nt; h2.sC = tt;
-            h2.alpha = 12.0; h2.beta = 1.0; h2.batch = nb;
+            h2.alpha = -12.0; h2.beta = 1.0; h2.batch = nb;
             gemm(h2, st);
         }
         gj(b.U.p, Rout, Nt, true);
+    };
+    // Tangent of one ratio step (after frp_step(n) has left Rinv_n in Rn and
+    // -Sinv B D^{-1} in Gt):  with G = W_n^{-1}[:, psi] = [Sinv; -D^{-1} B^T Sinv],
+    //   dRinv_n = Rinv_n (dRinv_{n-1} + wF_n G G^T) Rinv_n,  symmetrised.
+    // H holds dRinv_{n-1} on entry and is overwritten; U is free scratch.
+    auto frp_tangent = [&](int n, const double* Sv, const double* Rn, double* H, double* dRout) {
+        const int64_t sG = static_cast<int64_t>(Np) * Nt;
+        axpbyi(b.Gt.p, Np, sG, Sv, Np, pp, 1.0, nullptr, 0, 0, 0.0, 0.0, Np, Np, nb, st);   // Gt[:, psi] = Sinv
+        const bool ex = n < s->n_tr && Nf > 0;
+        Gemm g;   // H += wF_n G G^T  (psi-psi block only when G_f = 0)
+        g.M = ex ? Nt : Np; g.N = ex ? Nt : Np; g.K = Np;
+        g.A = b.Gt.p; g.lda = Np; g.sA = sG; g.tA = 1;
+        g.B = b.Gt.p; g.ldb = Np; g.sB = sG;
+        g.C = H; g.ldc = Nt; g.sC = tt; g.beta = 1.0; g.alpha_b = b.wF.p + static_cast<size_t>(n) * nb; g.batch = nb;
+        gemm(g, st);
+        Gemm g2;  // U = H Rinv_n
+        g2.M = Nt; g2.N = Nt; g2.K = Nt;
+        g2.A = H; g2.lda = Nt; g2.sA = tt; g2.B = Rn; g2.ldb = Nt; g2.sB = tt;
+        g2.C = b.U.p; g2.ldc = Nt; g2.sC = tt; g2.batch = nb;
+        gemm(g2, st);
+        Gemm g3;  // dRinv_n = Rinv_n U
+        g3.M = Nt; g3.N = Nt; g3.K = Nt;
+        g3.A = Rn; g3.lda = Nt; g3.sA = tt; g3.B = b.U.p; g3.ldb = Nt; g3.sB = tt;
+        g3.C = dRout; g3.ldc = Nt; g3.sC = tt; g3.batch = nb;
+        gemm(g3, st);
+        symmetrize(dRout, Nt, nb, st);
     };
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw1 = wall();
@@ -479,6 +637,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     // ---------------- forward sweep ----------------
     if (ns0 >= 2) diag_matrix(b.Rp.p, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
     else fill(b.Rp.p, tt * nb, 0.0, st);
+    if (tan) {
+        if (ns0 >= 2) diag_matrix(b.dRp.p, b.danl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
+        else fill(b.dRp.p, tt * nb, 0.0, st);
+    }
     // Sinv_n goes to ring slot n & 1, one step ahead of its consumer: the
     // side stream waits until FRP(n-2) has read the slot; FRP(n) waits for
     // Sinv_n.
@@ -493,15 +655,22 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
         if (n + 1 <= Nr - 2) sinv_ahead(n + 1);
         const int slot = n & 1;
         SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
-        frp_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb, b.Rp.p, b.Rc.p);
+        const double* Sslot = b.SinvR.p + static_cast<size_t>(slot) * pp * nb;
+        frp_step(n, Sslot, b.Rp.p, b.Rc.p);
+        if (tan) frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
         std::swap(b.Rp.p, b.Rc.p);
+        if (tan) std::swap(b.dRp.p, b.dRc.p);
         progress("forward sweep n =", n, Nr - 2);
         if ((n + 1) % L == 0) {
             const int sidx = (n + 1) / L;
-            if (sidx < n_seg)
+            if (sidx < n_seg) {
                 SEHF_HIP_CHECK(hipMemcpyAsync(b.ckpt.p + static_cast<size_t>(sidx) * nb * tt, b.Rp.p, nb * tt * sizeof(double),
                                               hipMemcpyDeviceToDevice, st));
+                if (tan)
+                    SEHF_HIP_CHECK(hipMemcpyAsync(b.dckpt.p + static_cast<size_t>(sidx) * nb * tt, b.dRp.p,
+                                                  nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+            }
         }
     }
     check_gj_info(b.gj, nb, st, "scan forward sweep");
@@ -549,52 +718,127 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
         gemm(g, st);
     }
     symmetrize(b.Kd.p, Np, nb, st);
+    if (tan) {
+        // S_schur = ((Rinv_{N-1})_psi,psi)^{-1}, so dS = -S (dRinv_{N-1})_psi,psi S;
+        // dK = sym(Bmat^{-1} (dAmat - dBmat K))
+        get_block(b.Q1.p, b.dRp.p, Nt, tt, 0, 0, Np, Np, nb, st);
+        Gemm g;   // Q2 = S dRinv_psi,psi
+        g.M = Np; g.N = Np; g.K = Np;
+        g.A = b.T.p; g.lda = Np; g.sA = pp; g.B = b.Q1.p; g.ldb = Np; g.sB = pp;
+        g.C = b.Q2.p; g.ldc = Np; g.sC = pp; g.batch = nb;
+        gemm(g, st);
+        Gemm g2;  // Q1 = dS = -Q2 S
+        g2.M = Np; g2.N = Np; g2.K = Np;
+        g2.A = b.Q2.p; g2.lda = Np; g2.sA = pp; g2.B = b.T.p; g2.ldb = Np; g2.sB = pp;
+        g2.C = b.Q1.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.batch = nb;
+        gemm(g2, st);
+        kme_dmats(b.dAm.p, b.dBm.p, b.Q1.p, b.T.p, b.tab.p, b.dtab.p, Np, nb, st);
+        Gemm g3;  // dAmat -= dBmat K
+        g3.M = Np; g3.N = Np; g3.K = Np;
+        g3.A = b.dBm.p; g3.lda = Np; g3.sA = pp; g3.B = b.Kd.p; g3.ldb = Np; g3.sB = pp;
+        g3.C = b.dAm.p; g3.ldc = Np; g3.sC = pp; g3.alpha = -1.0; g3.beta = 1.0; g3.batch = nb;
+        gemm(g3, st);
+        Gemm g4;  // dK = Bmat^{-1} (dAmat - dBmat K)
+        g4.M = Np; g4.N = Np; g4.K = Np;
+        g4.A = b.Psi.p; g4.lda = Np; g4.sA = pp; g4.B = b.dAm.p; g4.ldb = Np; g4.sB = pp;
+        g4.C = b.dKd.p; g4.ldc = Np; g4.sC = pp; g4.batch = nb;
+        gemm(g4, st);
+        symmetrize(b.dKd.p, Np, nb, st);
+    }
     // ---------------- outer boundary (BackPropagator.cpp:128-197, :538-551) ----------------
     psi_boundary(b.P.p, b.Kd.p, b.jy.p, Np, nb, st);
-    {
+    // Z_psi = psi + h^2/6 (E psi - V_N psi) + B_N f_N with f_N = -D_N^{-1} B_N^T psi, Z_f = 0
+    auto zmap = [&](const double* Pin, double* Zout) {
         Gemm g;   // Vpsi = V_N psi_N
         g.M = Np; g.N = Np; g.K = Np;
-        g.A = s->VN.p; g.lda = Np; g.sA = 0; g.B = b.P.p; g.ldb = Np; g.sB = pp;
+        g.A = s->VN.p; g.lda = Np; g.sA = 0; g.B = Pin; g.ldb = Np; g.sB = pp;
         g.C = b.Vpsi.p; g.ldc = Np; g.sC = pp; g.batch = nb;
         gemm(g, st);
+        if (Nf > 0 && N < s->n_tr) {
+            Gemm g1;  // G = B_N^T psi_N
+            g1.M = Nf; g1.N = Np; g1.K = Np;
+            g1.A = s->BN.p; g1.lda = Np; g1.sA = 0; g1.tA = 1; g1.B = Pin; g1.ldb = Np; g1.sB = pp;
+            g1.C = b.G.p; g1.ldc = Nf; g1.sC = pf; g1.batch = nb;
+            gemm(g1, st);
+            row_scale(b.G.p, b.G.p, s->negDinvN.p, Nf, Np, nb, st);   // f_N = -D^{-1} B^T psi
+            Gemm g2;  // Bf = B_N f_N
+            g2.M = Np; g2.N = Np; g2.K = Nf;
+            g2.A = s->BN.p; g2.lda = Np; g2.sA = 0; g2.B = b.G.p; g2.ldb = Nf; g2.sB = pf;
+            g2.C = b.Bf.p; g2.ldc = Np; g2.sC = pp; g2.batch = nb;
+            gemm(g2, st);
+        } else {
+            fill(b.Bf.p, pp * nb, 0.0, st);
+        }
+        z_boundary(Zout, Pin, b.Vpsi.p, b.Bf.p, b.cE.p, s->c6, Np, Nt, nb, st);
+    };
+    zmap(b.P.p, b.Z.p);
+    if (tan) {
+        // d psi_N from dK and the tables; dZ = zmap(d psi_N) + h^2/6 w_N psi_N
+        dpsi_boundary(b.dP.p, b.Kd.p, b.dKd.p, b.jy.p, b.djy.p, Np, nb, st);
+        zmap(b.dP.p, b.dZ.p);
+        axpy_b(b.dZ.p, Nt, static_cast<int64_t>(Nt) * Np, b.P.p, Np, pp, b.wN.p, 1.0, Np, Np, nb, st);
     }
-    if (Nf > 0 && N < s->n_tr) {
-        Gemm g;   // G = B_N^T psi_N
-        g.M = Nf; g.N = Np; g.K = Np;
-        g.A = s->BN.p; g.lda = Np; g.sA = 0; g.tA = 1; g.B = b.P.p; g.ldb = Np; g.sB = pp;
-        g.C = b.G.p; g.ldc = Nf; g.sC = pf; g.batch = nb;
-        gemm(g, st);
-        row_scale(b.G.p, b.G.p, s->negDinvN.p, Nf, Np, nb, st);   // f_N = -D^{-1} B^T psi
-        Gemm g2;  // Bf = B_N f_N
-        g2.M = Np; g2.N = Np; g2.K = Nf;
-        g2.A = s->BN.p; g2.lda = Np; g2.sA = 0; g2.B = b.G.p; g2.ldb = Nf; g2.sB = pf;
-        g2.C = b.Bf.p; g2.ldc = Np; g2.sC = pp; g2.batch = nb;
-        gemm(g2, st);
-    } else {
-        fill(b.Bf.p, pp * nb, 0.0, st);
-    }
-    z_boundary(b.Z.p, b.P.p, b.Vpsi.p, b.Bf.p, b.cE.p, s->c6, Np, Nt, nb, st);
     fill(b.acc.p, static_cast<size_t>(ns) * Np * nb, 0.0, st);
     fill(b.YJ.p, pp * nb, 0.0, st);
     fill(b.YY.p, pp * nb, 0.0, st);
-    auto dip_accum = [&](int n, const double* psi) {
+    auto dip_accum = [&](int n, const double* psi, double* acc) {
         Gemm g;   // acc += rows_n psi_n
         g.M = ns; g.N = Np; g.K = Np;
         g.A = s->Drows.p + static_cast<size_t>(n) * ns * Np; g.lda = ns; g.sA = 0;
         g.B = psi; g.ldb = Np; g.sB = pp;
-        g.C = b.acc.p; g.ldc = ns; g.sC = static_cast<int64_t>(ns) * Np;
+        g.C = acc; g.ldc = ns; g.sC = static_cast<int64_t>(ns) * Np;
         g.beta = 1.0; g.batch = nb;
         gemm(g, st);
     };
-    if (N < s->n_pts) dip_accum(N, b.P.p);
+    auto fit_tangent = [&](int n, const double* psi, const double* dpsi) {
+        const size_t o = static_cast<size_t>(n - n_fs) * Np;
+        fit_accum_t(b.dYJ.p, b.dYY.p, psi, dpsi, b.Jc.p + o, b.Yc.p + o, b.dJc.p + o, b.dYc.p + o, sJc, Np, nb, st);
+    };
+    if (N < s->n_pts) dip_accum(N, b.P.p, b.acc.p);
     if (N >= n_fs && N <= n_fe) fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(N - n_fs) * pp, sWin, b.P.p,
                                           b.Jc.p + static_cast<size_t>(N - n_fs) * Np, b.Yc.p + static_cast<size_t>(N - n_fs) * Np, sJc, Np, nb, st);
+    if (tan) {
+        fill(b.dacc.p, static_cast<size_t>(ns) * Np * nb, 0.0, st);
+        fill(b.dYJ.p, pp * nb, 0.0, st);
+        fill(b.dYY.p, pp * nb, 0.0, st);
+        if (N < s->n_pts) dip_accum(N, b.dP.p, b.dacc.p);
+        if (N >= n_fs && N <= n_fe) fit_tangent(N, b.P.p, b.dP.p);
+    }
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw3 = wall();
+    // psi-block of W_n^{-1} Zin (WInverseOperator.cpp:75-128):
+    //   out = Sinv_n (Z_psi - (B D^{-1}) Z_f)
+    const int64_t sZ = static_cast<int64_t>(Nt) * Np;
+    auto winv_psi = [&](int n, const double* Zin, const double* Sn, double* outp) {
+        get_block(b.T.p, Zin, Nt, sZ, 0, 0, Np, Np, nb, st);
+        if (n < s->n_tr && Nf > 0) {
+            Gemm g2;  // rhs = Z_psi - (B D^{-1}) Z_f
+            g2.M = Np; g2.N = Np; g2.K = Nf;
+            g2.A = s->BD.p + static_cast<size_t>(n) * pf; g2.lda = Np; g2.sA = 0;
+            g2.B = Zin + Np; g2.ldb = Nt; g2.sB = sZ;
+            g2.C = b.T.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.beta = 1.0; g2.batch = nb;
+            gemm(g2, st);
+        }
+        Gemm g3;  // out = Sinv_n rhs
+        g3.M = Np; g3.N = Np; g3.K = Np;
+        g3.A = Sn; g3.lda = Np; g3.sA = pp; g3.B = b.T.p; g3.ldb = Np; g3.sB = pp;
+        g3.C = outp; g3.ldc = Np; g3.sC = pp; g3.batch = nb;
+        gemm(g3, st);
+    };
+    auto mm = [&](double* C, const double* A, const double* Bm, int M, int Nn, int K, int64_t lda, int64_t sA,
+                  int64_t ldb, int64_t sB, int64_t ldc, int64_t sC, double beta) {
+        Gemm g;
+        g.M = M; g.N = Nn; g.K = K;
+        g.A = A; g.lda = lda; g.sA = sA; g.B = Bm; g.ldb = ldb; g.sB = sB;
+        g.C = C; g.ldc = ldc; g.sC = sC; g.beta = beta; g.batch = nb;
+        gemm(g, st);
+    };
     // ---------------- back-propagation by segments ----------------
     for (int sg = n_seg - 1; sg >= 0; --sg) {
         const int n0 = sg * L, n1 = std::min(n0 + L, N);
         const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
+        const double* dprev0 = (sg == 0 || !tan) ? nullptr : b.dckpt.p + static_cast<size_t>(sg) * nb * tt;
+        const double* dprev = dprev0;
         // the segment's Schur inverses on the side stream, after the previous
         // segment's backward products are done with segS
         if (sg != n_seg - 1) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
@@ -605,44 +849,70 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
         for (int n = n0; n < n1; ++n) {
             double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
             double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
+            double* dRn = tan ? b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt : nullptr;
             SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_seg[n - n0], 0));
             if (n >= ns0) {
                 frp_step(n, Sn, prev, Rn);
+                if (tan) {   // dRinv_{n-1} is kept: the tangent works on a copy
+                    SEHF_HIP_CHECK(hipMemcpyAsync(b.dRp.p, dprev, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    frp_tangent(n, Sn, Rn, b.dRp.p, dRn);
+                }
             } else if (n == 0) {
                 fill(Rn, tt * nb, 0.0, st);
+                if (tan) fill(dRn, tt * nb, 0.0, st);
             } else {
                 diag_matrix(Rn, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+                if (tan) diag_matrix(dRn, b.danl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
             }
             prev = Rn;
+            dprev = dRn;
         }
         progress("back-propagation segment", sg, n_seg);
         for (int n = n1 - 1; n >= n0; --n) {
             const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
             const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
-            Gemm g;   // Z <- Rinv_n Z
-            g.M = Nt; g.N = Np; g.K = Nt;
-            g.A = Rn; g.lda = Nt; g.sA = tt; g.B = b.Z.p; g.ldb = Nt; g.sB = static_cast<int64_t>(Nt) * Np;
-            g.C = b.Zt.p; g.ldc = Nt; g.sC = static_cast<int64_t>(Nt) * Np; g.batch = nb;
-            gemm(g, st);
+            mm(b.Zt.p, Rn, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);   // Z <- Rinv_n Z
             std::swap(b.Z.p, b.Zt.p);
-            get_block(b.T.p, b.Z.p, Nt, static_cast<int64_t>(Nt) * Np, 0, 0, Np, Np, nb, st);
-            if (n < s->n_tr && Nf > 0) {
-                Gemm g2;  // rhs = Z_psi - (B D^{-1}) Z_f  (WInverseOperator.cpp:103-112)
-                g2.M = Np; g2.N = Np; g2.K = Nf;
-                g2.A = s->BD.p + static_cast<size_t>(n) * pf; g2.lda = Np; g2.sA = 0;
-                g2.B = b.Z.p + Np; g2.ldb = Nt; g2.sB = static_cast<int64_t>(Nt) * Np;
-                g2.C = b.T.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.beta = 1.0; g2.batch = nb;
-                gemm(g2, st);
-            }
-            Gemm g3;  // psi_n = Sinv_n rhs
-            g3.M = Np; g3.N = Np; g3.K = Np;
-            g3.A = Sn; g3.lda = Np; g3.sA = pp; g3.B = b.T.p; g3.ldb = Np; g3.sB = pp;
-            g3.C = b.Psi.p; g3.ldc = Np; g3.sC = pp; g3.batch = nb;
-            gemm(g3, st);
-            if (n < s->n_pts) dip_accum(n, b.Psi.p);
+            winv_psi(n, b.Z.p, Sn, b.Psi.p);                                     // psi_n
+            if (n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
             if (n >= n_fs && n <= n_fe)
                 fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(n - n_fs) * pp, sWin, b.Psi.p,
                           b.Jc.p + static_cast<size_t>(n - n_fs) * Np, b.Yc.p + static_cast<size_t>(n - n_fs) * Np, sJc, Np, nb, st);
+            if (!tan) continue;
+            // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start
+            // dRinv_n = Rinv_n H_n Rinv_n, H_n = dRinv_{n-1} + wF_n G G^T, and
+            // G^T Z_n = psi_n, so
+            //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + wF_n G psi_n)
+            // with G psi_n = [P1; -D^{-1} B^T P1], P1 = Sinv_n psi_n.  This never
+            // multiplies by dRinv_n itself, which grows like 1/delta^2 where
+            // Rinv_n passes near a pole.
+            mm(b.P1.p, Sn, b.Psi.p, Np, Np, Np, Np, pp, Np, pp, Np, pp, 0.0);
+            const double* wFn = b.wF.p + static_cast<size_t>(n) * nb;
+            if (n >= ns0) {
+                const double* dRm1 = (n == n0) ? dprev0 : b.dsegR.p + static_cast<size_t>(n - 1 - n0) * nb * tt;
+                mm(b.dZ.p, dRm1, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);
+                axpy_b(b.dZ.p, Nt, sZ, b.P1.p, Np, pp, wFn, 1.0, Np, Np, nb, st);
+                if (n < s->n_tr && Nf > 0) {
+                    Gemm g;   // dZ_f -= wF_n (B D^{-1})^T P1
+                    g.M = Nf; g.N = Np; g.K = Np;
+                    g.A = s->BD.p + static_cast<size_t>(n) * pf; g.lda = Np; g.sA = 0; g.tA = 1;
+                    g.B = b.P1.p; g.ldb = Np; g.sB = pp;
+                    g.C = b.dZ.p + Np; g.ldc = Nt; g.sC = sZ;
+                    g.alpha = -1.0; g.alpha_b = wFn; g.beta = 1.0; g.batch = nb;
+                    gemm(g, st);
+                }
+                mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
+            } else {
+                const double* dRn = b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt;
+                mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
+                mm(b.dZt.p, dRn, b.Zt.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);   // Zt holds Z_{n+1}
+            }
+            std::swap(b.dZ.p, b.dZt.p);
+            // dpsi_n = (W_n^{-1} dZ_n)_psi + wP_n Sinv_n psi_n
+            winv_psi(n, b.dZ.p, Sn, b.dPsi.p);
+            axpy_b(b.dPsi.p, Np, pp, b.P1.p, Np, pp, b.wP.p + static_cast<size_t>(n) * nb, 1.0, Np, Np, nb, st);
+            if (n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
+            if (n >= n_fs && n <= n_fe) fit_tangent(n, b.Psi.p, b.dPsi.p);
         }
         SEHF_HIP_CHECK(hipEventRecord(g.ev_bwd, st));   // segS free for the next segment's Sinv
     }
@@ -651,6 +921,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     const double tw4 = wall();
     // ---------------- asymptotic fit ----------------
     fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
+    if (tan) fit_dsolve(b.dAm.p, b.dBm.p, b.YJ.p, b.YY.p, b.dYJ.p, b.dYY.p, b.coef.p, b.dcoef.p, Np, nb, st);
     fill(b.red.p, 2 * nb, 0.0, st);
     fit_resid(b.win.p, b.Am.p, b.Bm.p, b.Jc.p, b.Yc.p, b.coef.p, Np, n_fit, sWin, sJc, b.red.p, nb, st);
     SEHF_HIP_CHECK(hipMemcpyAsync(b.T.p, b.Am.p, nb * pp * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -680,6 +951,36 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     mgemm(b.ore.p, b.W2i.p, 1.0, b.acc.p + 6, ns, n_occ);
     mgemm(b.oim.p, b.W2i.p + Np, -1.0, b.acc.p + 6, ns, n_occ);
     dipole_final(b.Dre.p, b.Dim.p, b.DRre.p, b.DRim.p, b.ore.p, b.oim.p, s->dcorr.p, Np, n_occ, nb, st);
+    if (tan) {
+        // M = (C^{-1})^dagger with C = A - iB, so dM = -M dC^dagger M and, for
+        // y = M x, d(M x) = M v with v = dx - dC^dagger y, dC^dagger = dA^T + i dB^T:
+        //   v_re = dx - dA^T y_re + dB^T y_im,  v_im = -dB^T y_re - dA^T y_im;
+        //   M v = (P^T - i Q^T) v with P + iQ = C^{-1} (the first block column of W2i).
+        auto tg = [&](double* C, const double* A, int64_t lda, int64_t sA, const double* Bp, int ncol,
+                      double alpha, double beta) {
+            Gemm g;   // C (Np x ncol) = alpha A^T B + beta C
+            g.M = Np; g.N = ncol; g.K = Np;
+            g.A = A; g.lda = lda; g.sA = sA; g.tA = 1;
+            g.B = Bp; g.ldb = Np; g.sB = static_cast<int64_t>(Np) * ncol;
+            g.C = C; g.ldc = Np; g.sC = static_cast<int64_t>(Np) * ncol; g.alpha = alpha; g.beta = beta; g.batch = nb;
+            gemm(g, st);
+        };
+        auto mtan = [&](int r0, int ncol, const double* yre, const double* yim, double* ore_, double* oim_) {
+            const int64_t sv = static_cast<int64_t>(Np) * ncol;
+            transpose_scaled(b.vre.p, Np, sv, b.dacc.p + r0, ns, static_cast<int64_t>(ns) * Np, 1.0, ncol, Np, nb, st);
+            tg(b.vre.p, b.dAm.p, Np, pp, yre, ncol, -1.0, 1.0);
+            tg(b.vre.p, b.dBm.p, Np, pp, yim, ncol, 1.0, 1.0);
+            tg(b.vim.p, b.dBm.p, Np, pp, yre, ncol, -1.0, 0.0);
+            tg(b.vim.p, b.dAm.p, Np, pp, yim, ncol, -1.0, 1.0);
+            tg(ore_, b.W2i.p, w2ld, w2s, b.vre.p, ncol, 1.0, 0.0);
+            tg(ore_, b.W2i.p + Np, w2ld, w2s, b.vim.p, ncol, 1.0, 1.0);
+            tg(oim_, b.W2i.p, w2ld, w2s, b.vim.p, ncol, 1.0, 0.0);
+            tg(oim_, b.W2i.p + Np, w2ld, w2s, b.vre.p, ncol, -1.0, 1.0);
+        };
+        mtan(0, 6, b.DRre.p, b.DRim.p, b.dDRre.p, b.dDRim.p);
+        if (n_occ > 0) mtan(6, n_occ, b.ore.p, b.oim.p, b.dore.p, b.doim.p);
+        dipole_final(b.dDre.p, b.dDim.p, b.dDRre.p, b.dDRim.p, b.dore.p, b.doim.p, s->dcorr.p, Np, n_occ, nb, st);
+    }
     check_gj_info(b.gj, nb, st, "dipole normalisation");
     // ---------------- download ----------------
     auto dl = [&](double* dst, const double* src, size_t per) {
@@ -701,6 +1002,16 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
     std::vector<double> Ah(pp * nb), Kh(pp * nb);
     SEHF_HIP_CHECK(hipMemcpyAsync(Ah.data(), b.Am.p, Ah.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     SEHF_HIP_CHECK(hipMemcpyAsync(Kh.data(), b.Kfit.p, Kh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> dacc, ddre, ddim;
+    if (tan && tout) {
+        dl(tout->dK, b.dKd.p, pp);
+        dl(tout->dA, b.dAm.p, pp);
+        dl(tout->dB, b.dBm.p, pp);
+        dacc.resize(acc.size()); ddre.resize(dre.size()); ddim.resize(dre.size());
+        SEHF_HIP_CHECK(hipMemcpyAsync(dacc.data(), b.dacc.p, dacc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        SEHF_HIP_CHECK(hipMemcpyAsync(ddre.data(), b.dDre.p, ddre.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        SEHF_HIP_CHECK(hipMemcpyAsync(ddim.data(), b.dDim.p, ddim.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw5 = wall();
     g.t_stage[1] += tw2 - tw1; g.t_stage[2] += tw3 - tw2; g.t_stage[3] += tw4 - tw3;
@@ -720,6 +1031,20 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, int nb, int L, sehf_sca
             for (int al = 0; al < n_occ; ++al)
                 for (int be = 0; be < Np; ++be)
                     out->b_overlap[(static_cast<size_t>(E_) * n_occ + al) * Np + be] = a[6 + al + static_cast<size_t>(be) * ns];
+        if (tan && tout) {
+            const double* da = dacc.data() + static_cast<size_t>(e) * ns * Np;
+            for (int slot = 0; slot < 6; ++slot)
+                for (int be = 0; be < Np; ++be) {
+                    const size_t o = static_cast<size_t>(e) * 6 * Np + static_cast<size_t>(slot) * Np + be;
+                    const size_t od = (static_cast<size_t>(E_) * 6 + slot) * Np + be;
+                    if (tout->dd_raw) tout->dd_raw[od] = da[slot + static_cast<size_t>(be) * ns];
+                    if (tout->dD) { tout->dD[2 * od] = ddre[o]; tout->dD[2 * od + 1] = ddim[o]; }
+                }
+            if (tout->db_overlap)
+                for (int al = 0; al < n_occ; ++al)
+                    for (int be = 0; be < Np; ++be)
+                        tout->db_overlap[(static_cast<size_t>(E_) * n_occ + al) * Np + be] = da[6 + al + static_cast<size_t>(be) * ns];
+        }
         if (out->diag) {
             double* d = out->diag + static_cast<size_t>(E_) * 16;
             std::fill(d, d + 16, 0.0);
@@ -770,11 +1095,19 @@ int sehf_scan_info(const sehf_scan* s, int* info) {
     });
 }
 
-int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace {
+
+void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len, sehf_scan_out* out,
+              sehf_scan_tangent_out* tout) {
         if (nk <= 0) return;
+        if (!out) throw std::runtime_error("sehf_scan_run: out must not be NULL");
         for (int i = 0; i < nk; ++i)
             if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_run: every k must be > 0");
+        if (dir)
+            for (int i = 0; i < 2 * nk; ++i)
+                if (!std::isfinite(dir[i])) throw std::runtime_error("sehf_scan_run_tangent: directions must be finite");
         const int N = s->Nr - 1;
         int L = seg_len > 0 ? seg_len : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
         L = std::max(L, s->n_start + 1);
@@ -787,7 +1120,7 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
             s->groups.emplace_back(new Group());
             s->groups.back()->init();
         }
-        for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit);
+        for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, dir != nullptr);
         if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
         // batch i runs on group i % G; each group is driven by its own host thread
         std::vector<std::string> errs(G);
@@ -796,7 +1129,7 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
                 SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
                 for (int bi = gi; bi < n_batches; bi += G) {
                     const int e0 = bi * B, nb = std::min(B, nk - e0);
-                    run_batch(s, *s->groups[gi], k + e0, nb, L, out, e0);
+                    run_batch(s, *s->groups[gi], k + e0, dir ? dir + 2 * e0 : nullptr, nb, L, out, tout, e0);
                 }
             } catch (const std::exception& e) {
                 errs[gi] = e.what();
@@ -811,6 +1144,21 @@ int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len,
         }
         for (int gi = 0; gi < G; ++gi)
             if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out) {
+    return guarded([&] { scan_run(s, k, nullptr, nk, batch, seg_len, out, nullptr); });
+}
+
+int sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len,
+                          sehf_scan_out* out, sehf_scan_tangent_out* tout) {
+    return guarded([&] {
+        if (!dir || !tout) throw std::runtime_error("sehf_scan_run_tangent: dir and tout must not be NULL");
+        scan_run(s, k, dir, nk, batch, seg_len, out, tout);
     });
 }
 

@@ -11,7 +11,7 @@ namespace sehf {
 
 void init_U(double* U, const double* R, const double* Sinv, const double* Dinv, int Np, int Nt,
             int batch, hipStream_t s);
-void add_psi_f_blocks(double* U, const double* X, double a, int Np, int Nf, int Nt, int batch,
+void add_psi_f_blocks(double* U, const double* X, int64_t sX, double a, int Np, int Nf, int Nt, int batch,
                       hipStream_t s);
 void shift_copy(double* Y, const double* M0, const double* shift, int n, int batch, hipStream_t s);
 void sinv_matrix(double* Y, const double* S0, const double* cvd, const double* bdb, const double* cE,
@@ -38,5 +38,18 @@ void dipole_final(double* Dre, double* Dim, const double* DRre, const double* DR
                   const double* oim, const double* dcorr, int Np, int n_occ, int batch, hipStream_t s);
 void sumsq_block(const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r, int c, double* out,
                  int batch, hipStream_t s);
+
+// ---- tangent along (E, alpha) ----
+void axpy_b(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX,
+            const double* sc, double a, int r, int c, int batch, hipStream_t s);
+void kme_dmats(double* dAm, double* dBm, const double* dS, const double* S, const double* tab,
+               const double* dtab, int Np, int batch, hipStream_t s);
+void dpsi_boundary(double* dP, const double* K, const double* dK, const double* jy, const double* djy,
+                   int Np, int batch, hipStream_t s);
+void fit_accum_t(double* dYJ, double* dYY, const double* psi, const double* dpsi, const double* Jc,
+                 const double* Yc, const double* dJc, const double* dYc, int64_t sC, int Np, int batch,
+                 hipStream_t s);
+void fit_dsolve(double* dA, double* dB, const double* YJ, const double* YY, const double* dYJ,
+                const double* dYY, const double* coef, const double* dcoef, int Np, int batch, hipStream_t s);
 
 }  // namespace sehf

@@ -28,7 +28,7 @@ void init_U(double* U, const double* R, const double* Sinv, const double* Dinv, 
                        (int64_t)Nt * Nt, (int64_t)Np * Np);
 }
 
-// U_psi_f(i, j) += a X(i, j),  U_f_psi(j, i) += a X(i, j)   (X: Np x Nf)
+// U_psi_f(i, j) += a X(i, j),  U_f_psi(j, i) += a X(i, j)   (X: Np x Nf, leading dim Np)
 __global__ void k_add_psi_f(double* U, const double* X, double a, int Np, int Nf, int Nt, int64_t sU,
                             int64_t sX) {
     __shared__ double tile[32][33];
@@ -51,11 +51,10 @@ __global__ void k_add_psi_f(double* U, const double* X, double a, int Np, int Nf
         if (i < Np && j < Nf) Ub[(Np + j) + (int64_t)i * Nt] += tile[tx][yy];
     }
 }
-void add_psi_f_blocks(double* U, const double* X, double a, int Np, int Nf, int Nt, int batch,
+void add_psi_f_blocks(double* U, const double* X, int64_t sX, double a, int Np, int Nf, int Nt, int batch,
                       hipStream_t s) {
     dim3 g((Np + 31) / 32, (Nf + 31) / 32, batch);
-    hipLaunchKernelGGL(k_add_psi_f, g, dim3(256), 0, s, U, X, a, Np, Nf, Nt, (int64_t)Nt * Nt,
-                       (int64_t)Np * Nf);
+    hipLaunchKernelGGL(k_add_psi_f, g, dim3(256), 0, s, U, X, a, Np, Nf, Nt, (int64_t)Nt * Nt, sX);
 }
 
 // Per-energy diagonal shift of a shared matrix: Y_b = M0 + shift[b] I.
@@ -360,6 +359,115 @@ void sumsq_block(const double* X, int64_t ldx, int64_t sX, int i0, int j0, int r
                  int batch, hipStream_t s) {
     dim3 g((r + 255) / 256, c, batch);
     hipLaunchKernelGGL(k_sumsq, g, dim3(256), 0, s, X, ldx, sX, i0, j0, r, c, out);
+}
+
+// ---------------------------------------------------------------------------
+// Tangent (forward-mode derivative along the energy E and the polarisation
+// strength alpha) of the stage chain.  The O(N^3) work again runs in k_dgemm;
+// these kernels assemble the derivative operands.
+// ---------------------------------------------------------------------------
+
+// Y_b += a * s[b] * X_b   (r x c blocks; s may be null: factor 1).
+__global__ void k_axpy_b(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX,
+                         const double* sc, double a, int r, int c) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= r || j >= c) return;
+    const double f = sc ? a * sc[b] : a;
+    Y[b * sY + i + (int64_t)j * ldy] += f * X[b * sX + i + (int64_t)j * ldx];
+}
+void axpy_b(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX,
+            const double* sc, double a, int r, int c, int batch, hipStream_t s) {
+    dim3 g((r + 255) / 256, c, batch);
+    hipLaunchKernelGGL(k_axpy_b, g, dim3(256), 0, s, Y, ldy, sY, X, ldx, sX, sc, a, r, c);
+}
+
+// Derivatives of the K-matrix matching matrices (KMatrixExtractor.cpp:146-157)
+// with S = S_schur and dS its derivative:
+//   dAmat = d(W_out J_out) - dS (W_in J_in) - S d(W_in J_in)
+//   dBmat = dS (W_in N_in) + S d(W_in N_in) - d(W_out N_out)
+// dtab per energy: [4][Np] = d(W_out J_out), d(W_in J_in), d(W_in N_in), d(W_out N_out).
+__global__ void k_kme_dmats(double* dAm, double* dBm, const double* dS, const double* S, const double* tab,
+                            const double* dtab, int Np) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = i + (int64_t)j * Np;
+    const double* t = tab + b * 6 * Np;
+    const double* d = dtab + b * 4 * Np;
+    const double sv = S[b * s + o], dsv = dS[b * s + o];
+    dAm[b * s + o] = ((i == j) ? d[j] : 0.0) - dsv * (t[4 * Np + j] * t[j]) - sv * d[Np + j];
+    dBm[b * s + o] = dsv * (t[4 * Np + j] * t[2 * Np + j]) + sv * d[2 * Np + j] - ((i == j) ? d[3 * Np + j] : 0.0);
+}
+void kme_dmats(double* dAm, double* dBm, const double* dS, const double* S, const double* tab,
+               const double* dtab, int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_kme_dmats, g, dim3(256), 0, s, dAm, dBm, dS, S, tab, dtab, Np);
+}
+
+// d psi_N = dJ_N + dN_N K + N_N dK  (derivative of KMatrixExtractor.cpp:200-224);
+// djy per energy [2][Np] = dJ_N, dN_N.
+__global__ void k_dpsi_boundary(double* dP, const double* K, const double* dK, const double* jy,
+                                const double* djy, int Np) {
+    const int64_t b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = i + (int64_t)j * Np;
+    const double* t = jy + b * 2 * Np;
+    const double* d = djy + b * 2 * Np;
+    dP[b * s + o] = ((i == j) ? d[i] : 0.0) + d[Np + i] * K[b * s + o] + t[Np + i] * dK[b * s + o];
+}
+void dpsi_boundary(double* dP, const double* K, const double* dK, const double* jy, const double* djy,
+                   int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_dpsi_boundary, g, dim3(256), 0, s, dP, K, dK, jy, djy, Np);
+}
+
+// Fit sums' derivative at one window point:
+//   dYJ += dJc psi + Jc dpsi,  dYY += dYc psi + Yc dpsi   (row-wise tables).
+__global__ void k_fit_accum_t(double* dYJ, double* dYY, const double* psi, const double* dpsi,
+                              const double* Jc, const double* Yc, const double* dJc, const double* dYc,
+                              int Np, int64_t sM, int64_t sC) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (mu >= Np) return;
+    const int64_t o = mu + (int64_t)j * Np, t = b * sC + mu;
+    const double p = psi[b * sM + o], dp = dpsi[b * sM + o];
+    dYJ[b * sM + o] += dJc[t] * p + Jc[t] * dp;
+    dYY[b * sM + o] += dYc[t] * p + Yc[t] * dp;
+}
+void fit_accum_t(double* dYJ, double* dYY, const double* psi, const double* dpsi, const double* Jc,
+                 const double* Yc, const double* dJc, const double* dYc, int64_t sC, int Np, int batch,
+                 hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_fit_accum_t, g, dim3(256), 0, s, dYJ, dYY, psi, dpsi, Jc, Yc, dJc, dYc, Np,
+                       (int64_t)Np * Np, sC);
+}
+
+// Derivative of the per-row 2x2 fit solve (k_fit_solve):
+//   dA = dm00 YJ + m00 dYJ + dm01 YY + m01 dYY, dB likewise; closed rows 0.
+// dcoef per energy [4][Np] in coef's order.
+__global__ void k_fit_dsolve(double* dA, double* dB, const double* YJ, const double* YY, const double* dYJ,
+                             const double* dYY, const double* coef, const double* dcoef, int Np) {
+    const int64_t b = blockIdx.z;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (mu >= Np) return;
+    const int64_t s = (int64_t)Np * Np, o = mu + (int64_t)j * Np;
+    const double* c = coef + b * 5 * Np;
+    const double* d = dcoef + b * 4 * Np;
+    if (c[4 * Np + mu] != 0.0) {
+        dA[b * s + o] = 0.0;
+        dB[b * s + o] = 0.0;
+        return;
+    }
+    const double yj = YJ[b * s + o], yy = YY[b * s + o], dyj = dYJ[b * s + o], dyy = dYY[b * s + o];
+    dA[b * s + o] = d[mu] * yj + c[mu] * dyj + d[Np + mu] * yy + c[Np + mu] * dyy;
+    dB[b * s + o] = d[2 * Np + mu] * yj + c[2 * Np + mu] * dyj + d[3 * Np + mu] * yy + c[3 * Np + mu] * dyy;
+}
+void fit_dsolve(double* dA, double* dB, const double* YJ, const double* YY, const double* dYJ,
+                const double* dYY, const double* coef, const double* dcoef, int Np, int batch, hipStream_t s) {
+    dim3 g((Np + 255) / 256, Np, batch);
+    hipLaunchKernelGGL(k_fit_dsolve, g, dim3(256), 0, s, dA, dB, YJ, YY, dYJ, dYY, coef, dcoef, Np);
 }
 
 }  // namespace sehf

@@ -188,6 +188,21 @@ double Setup::polarization(double r) const {
     return 0.0;
 }
 
+double Setup::polarization_per_alpha(int ir) const {
+    const double r = ir * dr;
+    if (!(r > 1e-14) || r < 1.0e-6) return 0.0;   // V(ir) is zero at r = 0
+    if (P.pol_kind == 1) {
+        const double u = r / 1.5, u6 = u * u * u * u * u * u;
+        const double d = 1.0 - std::exp(-u6);
+        return -0.5 * d * d / (r * r * r * r);
+    }
+    if (P.pol_kind == 2) {
+        const double q = r * r + P.pol_rc * P.pol_rc;
+        return -1.0 / (2.0 * q * q);
+    }
+    return 0.0;
+}
+
 void Setup::potential(int ir, double* V) const {
     std::fill(V, V + static_cast<size_t>(Np) * Np, 0.0);
     const double r = ir * dr;

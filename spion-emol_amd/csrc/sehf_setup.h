@@ -39,6 +39,9 @@ public:
     void exchange_B(int ir, double* B) const;  // Np x Nf, col-major, (h^2/12) Q
     void dinv(int n, double* d) const;         // Nf: 1/max(1 - h^2/12 l(l+1)/r^2, W_min)
     double polarization(double r) const;
+    // d V_pol / d alpha at potential index ir (r = ir dr): the weight with
+    // which the polarisation strength enters the diagonal of V.
+    double polarization_per_alpha(int ir) const;
 
     // ---- dipole: per-radial-point weighted rows and the energy-independent
     // orthogonalisation integrals.

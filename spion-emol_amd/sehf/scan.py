@@ -31,6 +31,10 @@ class ScanOut_C(ctypes.Structure):
                                    "d_corr", "diag")]
 
 
+class TangentOut_C(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("dK", "dA", "dB", "db_overlap", "dd_raw", "dD")]
+
+
 def _register(L):
     if getattr(L, "_scan_registered", False):
         return L
@@ -41,6 +45,9 @@ def _register(L):
     L.sehf_scan_run.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ScanOut_C)]
     L.sehf_scan_run.restype = ctypes.c_int
+    L.sehf_scan_run_tangent.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ScanOut_C), ctypes.POINTER(TangentOut_C)]
+    L.sehf_scan_run_tangent.restype = ctypes.c_int
     L.sehf_scan_destroy.argtypes = [ctypes.c_void_p]
     L.sehf_scan_destroy.restype = None
     L._scan_registered = True
@@ -80,10 +87,15 @@ class Scan:
         check(L.sehf_scan_info(h, info))
         self.info = dict(zip(("n_psi", "n_f", "n_tot", "n_transition", "n_start", "n_pts", "Nr", "l_exch"), list(info)))
 
-    def run(self, ks, batch: int = 0, seg_len: int = 0, groups: int = 0):
+    def run(self, ks, batch: int = 0, seg_len: int = 0, groups: int = 0, tangent=None):
         """Solve every k.  `batch` energies advance through the radial grid
         together; `groups` (1..4) batches run concurrently on their own
-        streams (0 keeps the handle's current setting)."""
+        streams (0 keeps the handle's current setting).
+
+        tangent: None, one direction (t_E, t_alpha) for every energy, or an
+        [nk, 2] array of directions.  Each result then also holds the
+        derivatives dK, dA, dB, db_overlap, dd_raw and dD along its direction
+        (sehf_scan_run_tangent)."""
         L = load()
         if groups:
             check(L.sehf_scan_set_groups(self._h, int(groups)), "sehf_scan_set_groups")
@@ -95,15 +107,31 @@ class Scan:
              "d_corr": np.zeros((6, no)), "diag": np.zeros((nk, 16))}
         out = ScanOut_C(*[o[n].ctypes.data_as(_dp) for n in ("K", "A", "B", "K_fit", "b_overlap", "d_raw",
                                                              "D_raw", "D", "d_corr", "diag")])
-        check(L.sehf_scan_run(self._h, ks.ctypes.data_as(_dp), nk, batch, seg_len, ctypes.byref(out)),
-              "sehf_scan_run")
+        if tangent is None:
+            check(L.sehf_scan_run(self._h, ks.ctypes.data_as(_dp), nk, batch, seg_len, ctypes.byref(out)),
+                  "sehf_scan_run")
+        else:
+            d = np.asarray(tangent, dtype=np.float64)
+            d = np.ascontiguousarray(np.broadcast_to(d, (nk, 2)) if d.ndim == 1 else d)
+            if d.shape != (nk, 2):
+                raise ValueError(f"tangent directions must have shape ({nk}, 2), got {d.shape}")
+            o.update({"dK": np.zeros((nk, N, N)), "dA": np.zeros((nk, N, N)), "dB": np.zeros((nk, N, N)),
+                      "db_overlap": np.zeros((nk, no, N)), "dd_raw": np.zeros((nk, 6, N)),
+                      "dD": np.zeros((nk, 6, N), dtype=np.complex128)})
+            tout = TangentOut_C(*[o[n].ctypes.data_as(_dp) for n in ("dK", "dA", "dB", "db_overlap", "dd_raw", "dD")])
+            check(L.sehf_scan_run_tangent(self._h, ks.ctypes.data_as(_dp), d.ctypes.data_as(_dp), nk, batch,
+                                          seg_len, ctypes.byref(out), ctypes.byref(tout)), "sehf_scan_run_tangent")
         res = []
         for e in range(nk):
             # matrices come back column-major per energy
-            res.append({"k": float(ks[e]), "K": o["K"][e].T.copy(), "A": o["A"][e].T.copy(),
-                        "B": o["B"][e].T.copy(), "K_fit": o["K_fit"][e].T.copy(),
-                        "b_overlap": o["b_overlap"][e].T.copy(), "d_raw": o["d_raw"][e], "D_raw": o["D_raw"][e],
-                        "D": o["D"][e], "d_corr": o["d_corr"], "diag": o["diag"][e]})
+            r = {"k": float(ks[e]), "K": o["K"][e].T.copy(), "A": o["A"][e].T.copy(),
+                 "B": o["B"][e].T.copy(), "K_fit": o["K_fit"][e].T.copy(),
+                 "b_overlap": o["b_overlap"][e].T.copy(), "d_raw": o["d_raw"][e], "D_raw": o["D_raw"][e],
+                 "D": o["D"][e], "d_corr": o["d_corr"], "diag": o["diag"][e]}
+            if tangent is not None:
+                r.update({"dK": o["dK"][e].T.copy(), "dA": o["dA"][e].T.copy(), "dB": o["dB"][e].T.copy(),
+                          "db_overlap": o["db_overlap"][e].T.copy(), "dd_raw": o["dd_raw"][e], "dD": o["dD"][e]})
+            res.append(r)
         return res
 
     def stats(self) -> dict:
