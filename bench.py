@@ -43,6 +43,18 @@ def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
     return float(sinv + frp + bp + dip + 10 * Nt ** 3)
 
 
+def tangent_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
+    """Algorithmic FP64 flop the forward-mode derivative adds per energy (the
+    forward model's counting: no credit for the back-propagation recompute):
+    ratio steps: sum_{n>=n_start} G G^T 2Nt^2 Np + two products 4Nt^3
+    back-propagation: sum_n dRinv Z and Rinv(..) 4Nt^2 Np, Sinv psi and the
+        W^{-1} apply of dZ 4Np^3, [n<n_tr] 4Np^2 Nf ; dipole: n_tr (6+n_occ) 2Np^2."""
+    ex = min(n_tr, Nr)
+    fwd = (Nr - n_start) * (2 * Nt ** 2 * Np + 4 * Nt ** 3)
+    bp = Nr * (4 * Nt ** 2 * Np + 4 * Np ** 3) + ex * 4 * Np ** 2 * Nf
+    return float(fwd + bp + ex * (6 + n_occ) * 2 * Np ** 2)
+
+
 def cpu_baseline(p, ks, n_steps):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the
     host: one energy-independent setup, then `cores` energies concurrently
@@ -121,6 +133,9 @@ def main():
     ap.add_argument("--prof-every", type=int, default=97)
     ap.add_argument("--per-ik", action="store_true",
                     help="also time the plain per-ik path (batch 1, one group) on one energy of the full grid")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose batch width and group count with the library's autotuner (probe of radial steps) "
+                         "instead of --batch / --groups")
     ap.add_argument("--tangent", action="store_true",
                     help="time the batched scan with its forward-mode derivative along E (batched + adjoint)")
     ap.add_argument("--side-rmax", type=float, default=5.0,
@@ -159,6 +174,14 @@ def main():
 
     B = args.batch
     G = max(1, args.groups)
+    tune = None
+    if args.autotune:
+        t0 = time.time()
+        tune = S.autotune(ks_all[(np.arange(min(128, len(ks_all))) * 37) % len(ks_all)], probe_steps=8,
+                          tangent=args.tangent)
+        B, G = tune["batch"], tune["groups"]
+        tune["seconds"] = time.time() - t0
+        print(f"[bench] autotune: batch {B}, groups {G} ({tune['seconds']:.1f} s)", file=sys.stderr, flush=True)
     E_step = B * G
     nk = len(ks_all)
 
@@ -197,6 +220,9 @@ def main():
     value = energies / elapsed
     mflop = model_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
                                    info["n_transition"], info["n_start"], p.n_occ)
+    if args.tangent:
+        mflop += tangent_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
+                                          info["n_transition"], info["n_start"], p.n_occ)
     gemm_tflops = prof[1] / (prof[0] * 1e-3) / 1e12 if prof[0] > 0 else 0.0
     avg_launch_us = prof[0] / max(prof[2], 1) * 1e3
     buckets = {}
@@ -242,6 +268,8 @@ def main():
         t1 = time.time() - t0
         line["per_ik"] = {"value": 1.0 / t1, "unit": "energy points/s", "seconds": t1,
                           "batched_over_per_ik": value * t1}
+    if tune is not None:
+        line["autotune"] = tune
     if args.side_rmax > 0 and args.rmax == 0:
         line["side"] = side_measurements(args, p, ks_all, B)
     if not args.no_cpu:

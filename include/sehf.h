@@ -166,6 +166,19 @@ typedef struct {
  * energy i; the same k may appear twice with two directions. */
 int  sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len,
                            sehf_scan_out* out, sehf_scan_tangent_out* tout);
+/* Autotuner of the batch width and the number of concurrent groups.  For
+ * each candidate (batch in 1, 2, 4, ... 128 with groups * batch <= nk,
+ * groups in 1..2) that fits in device memory, the batch buffers are
+ * allocated and `probe_steps` radial steps of the forward recursion (Schur
+ * inverses and ratio steps at full size, with the tangent when `tangent` is
+ * set) are timed; a candidate list stops growing once the rate gains less
+ * than 3%.  The fastest candidate in energy-steps per second is returned and
+ * the handle's group count is set to it.  report[3*i .. 3*i+2] = (batch,
+ * groups, energy-steps/s) of each probe (at most 32); NULL allowed.  The
+ * batch buffers are released on return.  Replaces no reference interface:
+ * the reference runs one energy at a time. */
+int  sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, int tangent, int* batch_out,
+                        int* groups_out, double* report, int* n_report);
 /* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
  * fit + dipole, per-batch host tables. */
 int  sehf_scan_stats(const sehf_scan* s, double* t);

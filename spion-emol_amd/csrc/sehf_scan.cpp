@@ -66,6 +66,7 @@ struct Group {
     std::vector<hipEvent_t> ev_seg;
     Batch bt;
     double t_stage[8] = {0};
+    double t_probe = 0.0;   // seconds of the last probe run (autotuner)
     void init() {
         SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
@@ -335,8 +336,11 @@ double wall() {
 // One batch of energies through the whole path.  With `dir` (per energy the
 // direction (t_E, t_alpha)) the forward-mode derivative of every output along
 // t_E d/dE + t_alpha d/dalpha is propagated alongside (tout receives it).
+// probe > 0: time only the first `probe` steps of the forward sweep (Schur
+// inverses + ratio steps, with the tangent when dir is set) into
+// g.t_probe and return; nothing is written to out.
 void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int nb, int L, sehf_scan_out* out,
-               sehf_scan_tangent_out* tout, int out_off) {
+               sehf_scan_tangent_out* tout, int out_off, int probe = 0) {
     const double tw0 = wall();
     Setup& S = *s->S;
     hipStream_t st = g.st;
@@ -650,9 +654,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
     };
-    if (ns0 <= Nr - 2) sinv_ahead(ns0);
-    for (int n = ns0; n <= Nr - 2; ++n) {
-        if (n + 1 <= Nr - 2) sinv_ahead(n + 1);
+    const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1) : Nr - 2;
+    if (ns0 <= n_last) sinv_ahead(ns0);
+    for (int n = ns0; n <= n_last; ++n) {
+        if (n + 1 <= n_last) sinv_ahead(n + 1);
         const int slot = n & 1;
         SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
         const double* Sslot = b.SinvR.p + static_cast<size_t>(slot) * pp * nb;
@@ -672,6 +677,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                                                   nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
             }
         }
+    }
+    if (probe > 0) {
+        SEHF_HIP_CHECK(hipStreamSynchronize(st2));
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        g.t_probe = wall() - tw1;
+        return;
     }
     check_gj_info(b.gj, nb, st, "scan forward sweep");
     check_gj_info(b.gj2, nb, st2, "scan forward sweep (Schur inverses)");
@@ -1159,6 +1170,84 @@ int sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int 
     return guarded([&] {
         if (!dir || !tout) throw std::runtime_error("sehf_scan_run_tangent: dir and tout must not be NULL");
         scan_run(s, k, dir, nk, batch, seg_len, out, tout);
+    });
+}
+
+int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, int tangent, int* batch_out,
+                       int* groups_out, double* report, int* n_report) {
+    return guarded([&] {
+        if (nk <= 0 || !k) throw std::runtime_error("sehf_scan_autotune: empty k list");
+        for (int i = 0; i < nk; ++i)
+            if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_autotune: every k must be > 0");
+        const int steps = std::max(2, std::min(probe_steps > 0 ? probe_steps : 8, s->Nr - 1 - s->n_start));
+        const int N = s->Nr - 1;
+        int L = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
+        L = std::max(L, s->n_start + 1);
+        const int n_seg = (N + L - 1) / L, n_fit = 200;
+        std::vector<double> dir(tangent ? 2 * static_cast<size_t>(nk) : 0);
+        for (int i = 0; i < nk && tangent; ++i) { dir[2 * i] = 1.0; dir[2 * i + 1] = 0.0; }
+        const int cand[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+        double best = -1.0;
+        int bestB = 1, bestG = 1, nrep = 0;
+        for (int G = 1; G <= 2; ++G) {
+            double prev = -1.0;
+            for (int B : cand) {
+                if (G * B > nk) break;
+                s->groups.clear();   // release every batch buffer before sizing the next candidate
+                bool fits = true;
+                try {
+                    while (static_cast<int>(s->groups.size()) < G) {
+                        s->groups.emplace_back(new Group());
+                        s->groups.back()->init();
+                    }
+                    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, tangent != 0);
+                } catch (const std::exception&) {
+                    fits = false;
+                    (void)hipGetLastError();
+                }
+                if (!fits) break;
+                // energies of the probe: group gi takes k[(gi*B + e) % nk]
+                std::vector<std::vector<double>> kg(G, std::vector<double>(B)), dg(G, std::vector<double>(2 * B));
+                for (int gi = 0; gi < G; ++gi)
+                    for (int e = 0; e < B; ++e) {
+                        const int src = (gi * B + e) % nk;
+                        kg[gi][e] = k[src];
+                        if (tangent) { dg[gi][2 * e] = dir[2 * src]; dg[gi][2 * e + 1] = dir[2 * src + 1]; }
+                    }
+                double wall_s = 0.0;
+                for (int rep = 0; rep < 2; ++rep) {   // rep 0 warms the streams and kernels
+                    std::vector<std::string> errs(G);
+                    auto drive = [&](int gi) {
+                        try {
+                            SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
+                            run_batch(s, *s->groups[gi], kg[gi].data(), tangent ? dg[gi].data() : nullptr, B, L, nullptr,
+                                      nullptr, 0, rep == 0 ? 2 : steps);
+                        } catch (const std::exception& e) {
+                            errs[gi] = e.what();
+                        }
+                    };
+                    std::vector<std::thread> th;
+                    for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
+                    for (auto& x : th) x.join();
+                    for (int gi = 0; gi < G; ++gi)
+                        if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
+                    double tp = 0.0;
+                    for (int gi = 0; gi < G; ++gi) tp = std::max(tp, s->groups[gi]->t_probe);
+                    wall_s = std::max(tp, 1e-9);
+                }
+                const double rate = static_cast<double>(G) * B * steps / wall_s;   // energy-steps per second
+                if (report && nrep < 32) { report[3 * nrep] = B; report[3 * nrep + 1] = G; report[3 * nrep + 2] = rate; }
+                ++nrep;
+                if (rate > best) { best = rate; bestB = B; bestG = G; }
+                if (prev > 0.0 && rate < 1.03 * prev) break;   // saturated: larger batches only cost memory
+                prev = rate;
+            }
+        }
+        s->groups.clear();
+        s->n_groups = bestG;
+        if (batch_out) *batch_out = bestB;
+        if (groups_out) *groups_out = bestG;
+        if (n_report) *n_report = std::min(nrep, 32);
     });
 }
 

@@ -134,6 +134,20 @@ class Scan:
             res.append(r)
         return res
 
+    def autotune(self, ks, probe_steps: int = 8, tangent: bool = False) -> dict:
+        """Pick the batch width and group count for these energies by timing
+        probe_steps radial steps per candidate (sehf_scan_autotune); the
+        handle keeps the chosen group count."""
+        L = load()
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        b, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rep = (ctypes.c_double * 96)()
+        check(L.sehf_scan_autotune(self._h, ks.ctypes.data_as(_dp), len(ks), int(probe_steps), 1 if tangent else 0,
+                                   ctypes.byref(b), ctypes.byref(g), rep, ctypes.byref(n)), "sehf_scan_autotune")
+        table = [{"batch": int(rep[3 * i]), "groups": int(rep[3 * i + 1]), "energy_steps_per_s": rep[3 * i + 2]}
+                 for i in range(n.value)]
+        return {"batch": b.value, "groups": g.value, "probes": table}
+
     def stats(self) -> dict:
         t = (ctypes.c_double * 8)()
         check(load().sehf_scan_stats(self._h, t))

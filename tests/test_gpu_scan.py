@@ -119,3 +119,27 @@ def test_scan_polarisation_cfg4_shape(ctx):
     got = scan.solve_scan(p, ks, ctx=ctx)
     ref = O.solve_scan(p, ks)
     _compare(p, ks, got, ref, label="h2o_l4_pol")
+
+
+def test_autotune_then_run(ctx):
+    """The autotuner probes batch widths and group counts that fit, returns
+    the fastest, releases its buffers, and the scan then runs with the choice
+    and gives the same bits as any other split."""
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=4, dr=0.02, r_max=14.0, n_occ=3, seed=1)
+    ks = np.array([0.45, 0.7, 0.95, 1.1, 1.4, 0.6, 0.8, 1.2])
+    S = scan.Scan(p, ctx)
+    ref = S.run(ks, batch=8, groups=1)
+    tune = S.autotune(ks, probe_steps=6)
+    print(tune)
+    assert tune["probes"], tune
+    assert 1 <= tune["groups"] <= 2 and 1 <= tune["batch"] <= len(ks)
+    assert tune["batch"] * tune["groups"] <= len(ks)
+    best = max(tune["probes"], key=lambda r: r["energy_steps_per_s"])
+    assert (best["batch"], best["groups"]) == (tune["batch"], tune["groups"])
+    got = S.run(ks, batch=tune["batch"])
+    for a, b in zip(ref, got):
+        for key in ("K", "A", "B", "D"):
+            assert np.array_equal(a[key], b[key]), (key, a["k"])
+    tt = S.autotune(ks[:4], probe_steps=4, tangent=True)
+    assert tt["probes"] and tt["batch"] * tt["groups"] <= 4
