@@ -1,8 +1,9 @@
 // sehf_kernels.hip -- gfx950 FP64 kernels of the static-exchange solver.
 //
-//  * k_dgemm       : batched FP64 GEMM on v_mfma_f64_16x16x4f64, 64x64 output
-//                    tile per 256-thread workgroup (4 waves x 32x32), K staged
-//                    through LDS in slabs of 16.
+//  * k_dgemm       : batched FP64 GEMM on v_mfma_f64_16x16x4f64, 128x128 output
+//                    tile per 8-wave workgroup for deep products, 64x64 per
+//                    4 waves for thin ones; K staged through LDS in slabs of
+//                    16; XCD-aware tile order.
 //  * Gauss-Jordan  : in-place inversion with partial pivoting, recursive over
 //                    column blocks.  Leaves (<=16 columns) are factored by one
 //                    1024-thread workgroup per matrix with the panel held in
@@ -31,7 +32,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // GEMM
 // ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBK = 16;
 
 struct GemmK {
     int M, N, K;
@@ -43,33 +44,56 @@ struct GemmK {
     int zc0, zc1;
 };
 
-__global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
-    const int64_t b = blockIdx.z;
+// BM x BN output tile per workgroup of WM x WN waves; each wave owns a
+// (BM/WM) x (BN/WN) sub-tile of 16 x 16 MFMA blocks.  K runs in slabs of 16
+// staged through LDS with the next slab's global loads in flight.
+// Workgroups are dispatched round-robin over the 8 XCDs; the linear block
+// index is remapped so that each XCD works on one contiguous range of tiles
+// (neighbouring tiles of the same matrix share A row- and B column-panels in
+// that XCD's L2).
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_dgemm(GemmK p) {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
+    constexpr int RA = BM * GBK / NT, RB = BN * GBK / NT;
+    static_assert(TM % 16 == 0 && TN % 16 == 0 && RA * NT == BM * GBK && RB * NT == BN * GBK, "tile");
+    int bx, by, bz;
+    {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int T = gx * gy * gridDim.z;
+        const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const int x = L % 8, idx = L / 8, q = T / 8, r = T % 8;
+        const int nl = (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+        bx = nl % gx; by = (nl / gx) % gy; bz = nl / (gx * gy);
+    }
+    const int64_t b = bz;
     const double* __restrict__ A = p.A + b * p.sA;
     const double* __restrict__ B = p.B + b * p.sB;
     double* __restrict__ C = p.C + b * p.sC;
-    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
-    __shared__ double As[GBK][GBM + 2];   // [k][i] of op(A)
-    __shared__ double Bs[GBK][GBN + 2];   // [k][j] of op(B)
+    const int m0 = bx * BM, n0 = by * BN;
+    __shared__ double As[GBK][BM + 2];   // [k][i] of op(A)
+    __shared__ double Bs[GBK][BN + 2];   // [k][j] of op(B)
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
-    d4 acc[2][2];
+    const int wm = (wv % WM) * TM, wn = (wv / WM) * TN;
+    d4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    double ra[4], rb[4];
-    // Element assignment of the 64x16 slabs: 4 loads per thread, coalesced
-    // along the contiguous dimension of the stored operand.
+        for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[RA], rb[RB];
+    // Element assignment of the slabs: coalesced along the contiguous
+    // dimension of the stored operand.
     auto a_ik = [&](int r, int& i, int& k) {
-        if (!p.tA) { i = t & 63; k = (t >> 6) + 4 * r; } else { k = t & 15; i = (t >> 4) + 16 * r; }
+        const int e = t + r * NT;
+        if (!p.tA) { i = e % BM; k = e / BM; } else { k = e % GBK; i = e / GBK; }
     };
     auto b_kj = [&](int r, int& k, int& j) {
-        if (!p.tB) { k = t & 15; j = (t >> 4) + 16 * r; } else { j = t & 63; k = (t >> 6) + 4 * r; }
+        const int e = t + r * NT;
+        if (!p.tB) { k = e % GBK; j = e / GBK; } else { j = e % BN; k = e / BN; }
     };
     auto load = [&](int k0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < RA; ++r) {
             int i, k; a_ik(r, i, k);
             const int gi = m0 + i, gk = k0 + k;
             double v = 0.0;
@@ -77,7 +101,7 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
             ra[r] = v;
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < RB; ++r) {
             int k, j; b_kj(r, k, j);
             const int gj = n0 + j, gk = k0 + k;
             double v = 0.0;
@@ -89,36 +113,36 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmK p) {
     for (int k0 = 0; k0 < p.K; k0 += GBK) {
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 4; ++r) { int i, k; a_ik(r, i, k); As[k][i] = ra[r]; }
+        for (int r = 0; r < RA; ++r) { int i, k; a_ik(r, i, k); As[k][i] = ra[r]; }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) { int k, j; b_kj(r, k, j); Bs[k][j] = rb[r]; }
+        for (int r = 0; r < RB; ++r) { int k, j; b_kj(r, k, j); Bs[k][j] = rb[r]; }
         __syncthreads();
         if (k0 + GBK < p.K) load(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
             const int kq = kk + (lane >> 4);
-            double av[2], bv[2];
+            double av[FM], bv[FN];
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi) av[bi] = As[kq][wm + bi * 16 + (lane & 15)];
+            for (int bi = 0; bi < FM; ++bi) av[bi] = As[kq][wm + bi * 16 + (lane & 15)];
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) bv[bj] = Bs[kq][wn + bj * 16 + (lane & 15)];
+            for (int bj = 0; bj < FN; ++bj) bv[bj] = Bs[kq][wn + bj * 16 + (lane & 15)];
             // MFMA computes the transposed 16x16 block: its A operand is the
             // op(B) column slab, its B operand the op(A) row slab.  Result
             // register v of lane l holds D(4v + l/16, l%16), i.e.
             // C(row l%16, col 4v + l/16), so stores run along C's contiguous
             // row index.
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+            for (int bi = 0; bi < FM; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
+                for (int bj = 0; bj < FN; ++bj)
                     acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[bj], av[bi], acc[bi][bj], 0, 0, 0);
         }
     }
     const double al = p.alpha_b ? p.alpha * p.alpha_b[b] : p.alpha;
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < FM; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < FN; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm + bi * 16 + (lane & 15);
@@ -230,8 +254,18 @@ void gemm(const Gemm& g, hipStream_t s) {
             g.alpha, g.beta, g.alpha_b, g.zc0, g.zc1};
     // sampled timing buckets by depth: 2 (K < 64), 3 (64 <= K < 256), 4 (K >= 256)
     ProfScope ps(s, 2.0 * g.M * g.N * static_cast<double>(g.K) * g.batch, g.K < 64 ? 2 : (g.K < 256 ? 3 : 4));
-    dim3 grid((g.M + GBM - 1) / GBM, (g.N + GBN - 1) / GBN, g.batch);
-    hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, p);
+    // Tile by shape (measured on the Gauss-Jordan updates and back-propagation
+    // products at batch 32): 128 x 128 over 8 waves (64 x 32 each) for deep
+    // products that fill it (45-51 TFLOP/s where 64 x 64 gives 36-42);
+    // 64 x 64 over 4 waves for thin or shallow ones, which are bound by
+    // latency and memory and want the larger grid.
+    if (g.M > 64 && g.K >= 128) {
+        dim3 grid((g.M + 127) / 128, (g.N + 127) / 128, g.batch);
+        hipLaunchKernelGGL((k_dgemm<128, 128, 2, 4>), grid, dim3(512), 0, s, p);
+    } else {
+        dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, g.batch);
+        hipLaunchKernelGGL((k_dgemm<64, 64, 2, 2>), grid, dim3(256), 0, s, p);
+    }
 }
 
 // ---------------------------------------------------------------------------

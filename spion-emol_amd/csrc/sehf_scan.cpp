@@ -11,8 +11,9 @@
 // (BackPropagator.cpp:603-703) walks the grid downwards one segment at a
 // time, recomputing that segment's Sinv and R from its checkpoint, and folds
 // the dipole integrands (DipoleMatrixElement.cpp:584-965) and the asymptotic
-// fit sums (AsymptoticAmplitudes.cpp:64-359) into the sweep, so psi is never
-// stored except for the ~200-point fit window.
+// fit residual (AsymptoticAmplitudes.cpp:64-359) into the sweep, so psi is
+// never stored; the fit sums over the ~200-point window at the top of the
+// grid come from a shorter pre-pass over the top segments.
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
@@ -44,7 +45,7 @@ namespace {
 struct Batch {
     int B = 0;   // capacity
     DevArray<double> Rp, Rc, U, Ms, X, Z, Zt, T, Psi, acc, YJ, YY, Am, Bm, Kd, P, Vpsi, G, Bf;
-    DevArray<double> ckpt, segR, segS, win, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
+    DevArray<double> ckpt, segR, segS, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
     DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
     GjBuffers gj;             // inversions on the main stream
@@ -276,7 +277,7 @@ void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_ou
 void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, bool tan) {
     Batch& b = g.bt;
     if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_seg, 1)) * B * s->Nt * s->Nt &&
-        b.win.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np * s->Np && (!tan || b.has_tan))
+        b.Jc.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np && (!tan || b.has_tan))
         return;
     const size_t Np = s->Np, Nf = s->Nf, Nt = s->Nt, ns = s->nslot;
     const size_t tt = Nt * Nt, pp = Np * Np, pf = Np * std::max<size_t>(Nf, 1), ff = std::max<size_t>(Nf, 1) * std::max<size_t>(Nf, 1);
@@ -290,7 +291,6 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, boo
     b.P.alloc(pp * B); b.Vpsi.alloc(pp * B); b.G.alloc(pf * B); b.Bf.alloc(pp * B);
     b.ckpt.alloc(tt * B * std::max(n_seg, 1));
     b.segR.alloc(tt * B * L); b.segS.alloc(pp * B * L);
-    b.win.alloc(pp * B * std::max(n_fit, 1));
     b.W2.alloc(4 * pp * B); b.W2i.alloc(4 * pp * B);
     b.Rff.alloc(ff * B); b.Rffi.alloc(ff * B); b.T1.alloc(pf * B);
     b.DRre.alloc(6 * Np * B); b.DRim.alloc(6 * Np * B); b.ore.alloc(Np * s->S->n_occ * B); b.oim.alloc(Np * s->S->n_occ * B);
@@ -561,7 +561,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     hipStream_t st2 = g.st2;
     SEHF_HIP_CHECK(hipEventRecord(g.ev_in, st));
     SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_in, 0));
-    const int64_t sJc = static_cast<int64_t>(n_fit) * Np, sWin = static_cast<int64_t>(n_fit) * pp;
+    const int64_t sJc = static_cast<int64_t>(n_fit) * Np;
 
     auto gj = [&](double* mat, double* outp, int n, bool sym) {
         gj_inverse(mat, outp, n, n, static_cast<int64_t>(n) * n, nb, sym, b.gj.work(), st);
@@ -806,8 +806,19 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         fit_accum_t(b.dYJ.p, b.dYY.p, psi, dpsi, b.Jc.p + o, b.Yc.p + o, b.dJc.p + o, b.dYc.p + o, sJc, Np, nb, st);
     };
     if (N < s->n_pts) dip_accum(N, b.P.p, b.acc.p);
-    if (N >= n_fs && N <= n_fe) fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(N - n_fs) * pp, sWin, b.P.p,
-                                          b.Jc.p + static_cast<size_t>(N - n_fs) * Np, b.Yc.p + static_cast<size_t>(N - n_fs) * Np, sJc, Np, nb, st);
+    // The fit window [n_fs, n_fe] ends 3 points below the boundary N.  Its
+    // sums YJ, YY come from a pre-pass over the top segments (below), so the
+    // main pass knows A and B and takes the fit residual point by point; no
+    // copy of the window's psi is kept.
+    auto fit_sums = [&](int n, const double* psi) {
+        const size_t o = static_cast<size_t>(n - n_fs) * Np;
+        fit_accum(b.YJ.p, b.YY.p, nullptr, 0, psi, b.Jc.p + o, b.Yc.p + o, sJc, Np, nb, st);
+    };
+    auto fit_resid_at = [&](int n, const double* psi) {
+        const size_t o = static_cast<size_t>(n - n_fs) * Np;
+        fit_resid(psi, b.Am.p, b.Bm.p, b.Jc.p + o, b.Yc.p + o, b.coef.p, Np, 1, static_cast<int64_t>(pp), sJc, b.red.p,
+                  nb, st);
+    };
     if (tan) {
         fill(b.dacc.p, static_cast<size_t>(ns) * Np * nb, 0.0, st);
         fill(b.dYJ.p, pp * nb, 0.0, st);
@@ -815,6 +826,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (N < s->n_pts) dip_accum(N, b.dP.p, b.dacc.p);
         if (N >= n_fs && N <= n_fe) fit_tangent(N, b.P.p, b.dP.p);
     }
+    fill(b.red.p, 2 * nb, 0.0, st);
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw3 = wall();
     // psi-block of W_n^{-1} Zin (WInverseOperator.cpp:75-128):
@@ -845,96 +857,112 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         gemm(g, st);
     };
     // ---------------- back-propagation by segments ----------------
-    for (int sg = n_seg - 1; sg >= 0; --sg) {
-        const int n0 = sg * L, n1 = std::min(n0 + L, N);
-        const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
-        const double* dprev0 = (sg == 0 || !tan) ? nullptr : b.dckpt.p + static_cast<size_t>(sg) * nb * tt;
-        const double* dprev = dprev0;
-        // the segment's Schur inverses on the side stream, after the previous
-        // segment's backward products are done with segS
-        if (sg != n_seg - 1) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
-        for (int n = n0; n < n1; ++n) {
-            sinv_step(n, b.segS.p + static_cast<size_t>(n - n0) * nb * pp);
-            SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n - n0], st2));
-        }
-        for (int n = n0; n < n1; ++n) {
-            double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
-            double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
-            double* dRn = tan ? b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt : nullptr;
-            SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_seg[n - n0], 0));
-            if (n >= ns0) {
-                frp_step(n, Sn, prev, Rn);
-                if (tan) {   // dRinv_{n-1} is kept: the tangent works on a copy
-                    SEHF_HIP_CHECK(hipMemcpyAsync(b.dRp.p, dprev, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
-                    frp_tangent(n, Sn, Rn, b.dRp.p, dRn);
-                }
-            } else if (n == 0) {
-                fill(Rn, tt * nb, 0.0, st);
-                if (tan) fill(dRn, tt * nb, 0.0, st);
-            } else {
-                diag_matrix(Rn, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
-                if (tan) diag_matrix(dRn, b.danl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+    // sweep(main = false): the fit pre-pass, segments from the top down to the
+    // one holding n_fs, psi only, stopping at n_fs.  sweep(main = true): the
+    // whole grid with the dipole sums, the fit residual and the tangent.
+    bool seg_pending = false;   // a previous segment's backward products may still read segS
+    auto sweep = [&](bool main) {
+        const int sg_lo = main ? 0 : n_fs / L;
+        for (int sg = n_seg - 1; sg >= sg_lo; --sg) {
+            const int n0 = sg * L, n1 = std::min(n0 + L, N);
+            const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
+            const bool tg = main && tan;
+            const double* dprev0 = (sg == 0 || !tg) ? nullptr : b.dckpt.p + static_cast<size_t>(sg) * nb * tt;
+            const double* dprev = dprev0;
+            // the segment's Schur inverses on the side stream, after the
+            // previous segment's backward products are done with segS
+            if (seg_pending) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
+            for (int n = n0; n < n1; ++n) {
+                sinv_step(n, b.segS.p + static_cast<size_t>(n - n0) * nb * pp);
+                SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n - n0], st2));
             }
-            prev = Rn;
-            dprev = dRn;
-        }
-        progress("back-propagation segment", sg, n_seg);
-        for (int n = n1 - 1; n >= n0; --n) {
-            const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
-            const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
-            mm(b.Zt.p, Rn, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);   // Z <- Rinv_n Z
-            std::swap(b.Z.p, b.Zt.p);
-            winv_psi(n, b.Z.p, Sn, b.Psi.p);                                     // psi_n
-            if (n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
-            if (n >= n_fs && n <= n_fe)
-                fit_accum(b.YJ.p, b.YY.p, b.win.p + static_cast<size_t>(n - n_fs) * pp, sWin, b.Psi.p,
-                          b.Jc.p + static_cast<size_t>(n - n_fs) * Np, b.Yc.p + static_cast<size_t>(n - n_fs) * Np, sJc, Np, nb, st);
-            if (!tan) continue;
-            // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start
-            // dRinv_n = Rinv_n H_n Rinv_n, H_n = dRinv_{n-1} + wF_n G G^T, and
-            // G^T Z_n = psi_n, so
-            //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + wF_n G psi_n)
-            // with G psi_n = [P1; -D^{-1} B^T P1], P1 = Sinv_n psi_n.  This never
-            // multiplies by dRinv_n itself, which grows like 1/delta^2 where
-            // Rinv_n passes near a pole.
-            mm(b.P1.p, Sn, b.Psi.p, Np, Np, Np, Np, pp, Np, pp, Np, pp, 0.0);
-            const double* wFn = b.wF.p + static_cast<size_t>(n) * nb;
-            if (n >= ns0) {
-                const double* dRm1 = (n == n0) ? dprev0 : b.dsegR.p + static_cast<size_t>(n - 1 - n0) * nb * tt;
-                mm(b.dZ.p, dRm1, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);
-                axpy_b(b.dZ.p, Nt, sZ, b.P1.p, Np, pp, wFn, 1.0, Np, Np, nb, st);
-                if (n < s->n_tr && Nf > 0) {
-                    Gemm g;   // dZ_f -= wF_n (B D^{-1})^T P1
-                    g.M = Nf; g.N = Np; g.K = Np;
-                    g.A = s->BD.p + static_cast<size_t>(n) * pf; g.lda = Np; g.sA = 0; g.tA = 1;
-                    g.B = b.P1.p; g.ldb = Np; g.sB = pp;
-                    g.C = b.dZ.p + Np; g.ldc = Nt; g.sC = sZ;
-                    g.alpha = -1.0; g.alpha_b = wFn; g.beta = 1.0; g.batch = nb;
-                    gemm(g, st);
+            for (int n = n0; n < n1; ++n) {
+                double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
+                double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
+                double* dRn = tg ? b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt : nullptr;
+                SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_seg[n - n0], 0));
+                if (n >= ns0) {
+                    frp_step(n, Sn, prev, Rn);
+                    if (tg) {   // dRinv_{n-1} is kept: the tangent works on a copy
+                        SEHF_HIP_CHECK(hipMemcpyAsync(b.dRp.p, dprev, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+                        frp_tangent(n, Sn, Rn, b.dRp.p, dRn);
+                    }
+                } else if (n == 0) {
+                    fill(Rn, tt * nb, 0.0, st);
+                    if (tg) fill(dRn, tt * nb, 0.0, st);
+                } else {
+                    diag_matrix(Rn, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+                    if (tg) diag_matrix(dRn, b.danl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
                 }
-                mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
-            } else {
-                const double* dRn = b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt;
-                mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
-                mm(b.dZt.p, dRn, b.Zt.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);   // Zt holds Z_{n+1}
+                prev = Rn;
+                dprev = dRn;
             }
-            std::swap(b.dZ.p, b.dZt.p);
-            // dpsi_n = (W_n^{-1} dZ_n)_psi + wP_n Sinv_n psi_n
-            winv_psi(n, b.dZ.p, Sn, b.dPsi.p);
-            axpy_b(b.dPsi.p, Np, pp, b.P1.p, Np, pp, b.wP.p + static_cast<size_t>(n) * nb, 1.0, Np, Np, nb, st);
-            if (n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
-            if (n >= n_fs && n <= n_fe) fit_tangent(n, b.Psi.p, b.dPsi.p);
+            progress(main ? "back-propagation segment" : "fit pre-pass segment", sg, n_seg);
+            const int n_stop = main ? n0 : std::max(n0, n_fs);
+            for (int n = n1 - 1; n >= n_stop; --n) {
+                const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
+                const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
+                mm(b.Zt.p, Rn, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);   // Z <- Rinv_n Z
+                std::swap(b.Z.p, b.Zt.p);
+                winv_psi(n, b.Z.p, Sn, b.Psi.p);                                     // psi_n
+                const bool in_win = n >= n_fs && n <= n_fe;
+                if (!main) {
+                    if (in_win) fit_sums(n, b.Psi.p);
+                    continue;
+                }
+                if (n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
+                if (in_win) fit_resid_at(n, b.Psi.p);
+                if (!tan) continue;
+                // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start
+                // dRinv_n = Rinv_n H_n Rinv_n, H_n = dRinv_{n-1} + wF_n G G^T, and
+                // G^T Z_n = psi_n, so
+                //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + wF_n G psi_n)
+                // with G psi_n = [P1; -D^{-1} B^T P1], P1 = Sinv_n psi_n.  This never
+                // multiplies by dRinv_n itself, which grows like 1/delta^2 where
+                // Rinv_n passes near a pole.
+                mm(b.P1.p, Sn, b.Psi.p, Np, Np, Np, Np, pp, Np, pp, Np, pp, 0.0);
+                const double* wFn = b.wF.p + static_cast<size_t>(n) * nb;
+                if (n >= ns0) {
+                    const double* dRm1 = (n == n0) ? dprev0 : b.dsegR.p + static_cast<size_t>(n - 1 - n0) * nb * tt;
+                    mm(b.dZ.p, dRm1, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);
+                    axpy_b(b.dZ.p, Nt, sZ, b.P1.p, Np, pp, wFn, 1.0, Np, Np, nb, st);
+                    if (n < s->n_tr && Nf > 0) {
+                        Gemm g;   // dZ_f -= wF_n (B D^{-1})^T P1
+                        g.M = Nf; g.N = Np; g.K = Np;
+                        g.A = s->BD.p + static_cast<size_t>(n) * pf; g.lda = Np; g.sA = 0; g.tA = 1;
+                        g.B = b.P1.p; g.ldb = Np; g.sB = pp;
+                        g.C = b.dZ.p + Np; g.ldc = Nt; g.sC = sZ;
+                        g.alpha = -1.0; g.alpha_b = wFn; g.beta = 1.0; g.batch = nb;
+                        gemm(g, st);
+                    }
+                    mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
+                } else {
+                    const double* dRn = b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt;
+                    mm(b.dZt.p, Rn, b.dZ.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);
+                    mm(b.dZt.p, dRn, b.Zt.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 1.0);   // Zt holds Z_{n+1}
+                }
+                std::swap(b.dZ.p, b.dZt.p);
+                // dpsi_n = (W_n^{-1} dZ_n)_psi + wP_n Sinv_n psi_n
+                winv_psi(n, b.dZ.p, Sn, b.dPsi.p);
+                axpy_b(b.dPsi.p, Np, pp, b.P1.p, Np, pp, b.wP.p + static_cast<size_t>(n) * nb, 1.0, Np, Np, nb, st);
+                if (n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
+                if (in_win) fit_tangent(n, b.Psi.p, b.dPsi.p);
+            }
+            SEHF_HIP_CHECK(hipEventRecord(g.ev_bwd, st));   // segS free for the next segment's Sinv
+            seg_pending = true;
         }
-        SEHF_HIP_CHECK(hipEventRecord(g.ev_bwd, st));   // segS free for the next segment's Sinv
-    }
+    };
+    sweep(false);
+    // A, B of the fit from the completed window sums; the main pass then
+    // starts again from the outer boundary
+    fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
+    zmap(b.P.p, b.Z.p);
+    sweep(true);
     check_gj_info(b.gj, nb, st, "scan back-propagation");
     check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
     // ---------------- asymptotic fit ----------------
-    fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
     if (tan) fit_dsolve(b.dAm.p, b.dBm.p, b.YJ.p, b.YY.p, b.dYJ.p, b.dYY.p, b.coef.p, b.dcoef.p, Np, nb, st);
-    fill(b.red.p, 2 * nb, 0.0, st);
-    fit_resid(b.win.p, b.Am.p, b.Bm.p, b.Jc.p, b.Yc.p, b.coef.p, Np, n_fit, sWin, sJc, b.red.p, nb, st);
     SEHF_HIP_CHECK(hipMemcpyAsync(b.T.p, b.Am.p, nb * pp * sizeof(double), hipMemcpyDeviceToDevice, st));
     gj(b.T.p, b.Ainv.p, Np, false);
     check_gj_info(b.gj, nb, st, "fitted A matrix");

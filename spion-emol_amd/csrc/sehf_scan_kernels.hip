@@ -122,8 +122,8 @@ void diag_matrix(double* R, const double* d, int n, int batch, hipStream_t s) {
 }
 
 // Fit accumulation for window point i (AsymptoticAmplitudes.cpp:255-258):
-//   YJ += diag(Jc(i,:)) psi, YY += diag(Yc(i,:)) psi; the window copy of psi
-//   is kept for the residual pass.
+//   YJ += diag(Jc(i,:)) psi, YY += diag(Yc(i,:)) psi; with `win` the window
+//   copy of psi is kept as well.
 __global__ void k_fit_accum(double* YJ, double* YY, double* win, const double* psi, const double* Jc,
                             const double* Yc, int Np, int64_t sM, int64_t sWin, int64_t sC) {
     const int64_t b = blockIdx.z;
@@ -133,7 +133,7 @@ __global__ void k_fit_accum(double* YJ, double* YY, double* win, const double* p
     const double p = psi[b * sM + o];
     YJ[b * sM + o] += Jc[b * sC + mu] * p;
     YY[b * sM + o] += Yc[b * sC + mu] * p;
-    win[b * sWin + o] = p;
+    if (win) win[b * sWin + o] = p;
 }
 void fit_accum(double* YJ, double* YY, double* win, int64_t sWin, const double* psi, const double* Jc,
                const double* Yc, int64_t sC, int Np, int batch, hipStream_t s) {
@@ -171,7 +171,9 @@ __device__ inline void amax_pos(double* addr, double v) {
 }
 
 // Residual pass (AsymptoticAmplitudes.cpp:284-305): out[b*2+0] = max |fit - psi|
-// over open rows, out[b*2+1] = max |psi| over the window.
+// over open rows, out[b*2+1] = max |psi| over the nfit window points given
+// (points at stride Np^2, energies at stride sWin); the maxima accumulate
+// over calls.
 __global__ void k_fit_resid(const double* win, const double* A, const double* B, const double* Jc,
                             const double* Yc, const double* coef, int Np, int nfit, int64_t sWin,
                             int64_t sC, double* out) {
