@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -336,11 +337,32 @@ double wall() {
 // One batch of energies through the whole path.  With `dir` (per energy the
 // direction (t_E, t_alpha)) the forward-mode derivative of every output along
 // t_E d/dE + t_alpha d/dalpha is propagated alongside (tout receives it).
+// Start line of concurrent probes: every group's host thread prepares its
+// tables, then all start their timed steps together.
+struct ProbeSync {
+    std::mutex mu;
+    std::condition_variable cv;
+    int waiting = 0, total = 1;
+    double t_start = 0.0;
+    double arrive(double now) {
+        std::unique_lock<std::mutex> lk(mu);
+        if (waiting >= total) return t_start;   // already released (a failing thread arriving late)
+        if (++waiting == total) {
+            t_start = now;
+            cv.notify_all();
+        } else {
+            cv.wait(lk, [&] { return waiting >= total; });
+        }
+        return t_start;
+    }
+};
+
 // probe > 0: time only the first `probe` steps of the forward sweep (Schur
-// inverses + ratio steps, with the tangent when dir is set) into
-// g.t_probe and return; nothing is written to out.
+// inverses + ratio steps, with the tangent when dir is set) and return;
+// nothing is written to out.  g.t_probe = end of this group's steps minus the
+// common start of all groups (sync).
 void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int nb, int L, sehf_scan_out* out,
-               sehf_scan_tangent_out* tout, int out_off, int probe = 0) {
+               sehf_scan_tangent_out* tout, int out_off, int probe = 0, ProbeSync* sync = nullptr) {
     const double tw0 = wall();
     Setup& S = *s->S;
     hipStream_t st = g.st;
@@ -623,7 +645,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         symmetrize(dRout, Nt, nb, st);
     };
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
-    const double tw1 = wall();
+    const double tw1 = sync ? sync->arrive(wall()) : wall();
     // Optional progress lines on stderr (SEHF_PROGRESS=<seconds>): the stream
     // is drained before each line so the reported radial index has finished.
     static const double prog_every = [] {
@@ -1245,13 +1267,16 @@ int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, i
                 double wall_s = 0.0;
                 for (int rep = 0; rep < 2; ++rep) {   // rep 0 warms the streams and kernels
                     std::vector<std::string> errs(G);
+                    ProbeSync sync;
+                    sync.total = G;
                     auto drive = [&](int gi) {
                         try {
                             SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
                             run_batch(s, *s->groups[gi], kg[gi].data(), tangent ? dg[gi].data() : nullptr, B, L, nullptr,
-                                      nullptr, 0, rep == 0 ? 2 : steps);
+                                      nullptr, 0, rep == 0 ? 2 : steps, &sync);
                         } catch (const std::exception& e) {
                             errs[gi] = e.what();
+                            sync.arrive(wall());   // never leave the other groups waiting (no-op once all arrived)
                         }
                     };
                     std::vector<std::thread> th;
