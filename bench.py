@@ -102,7 +102,7 @@ def side_measurements(args, p, ks_all, B):
                      "per_ik_energies_per_s": 1.0 / t_1, "batched_over_per_ik": (B / t_b) * t_1}
     print(f"[bench] side: batched {B / t_b:.3f} vs per-ik {1.0 / t_1:.3f} energies/s on Nr={q.Nr}",
           file=sys.stderr, flush=True)
-    del S
+    S.close()
     pc, kc = synth.config("h2o_l20_pol")
     q = synth.h2o_like(pc.l_cont, pc.dr, args.side_rmax, pc.n_occ, seed=1, pol=(pc.pol_alpha, pc.pol_rc),
                        allow_closed=True)
@@ -115,6 +115,7 @@ def side_measurements(args, p, ks_all, B):
                       "forward_energies_per_s": B / t_f, "with_tangent_energies_per_s": B / t_t,
                       "tangent_over_forward_cost": t_t / t_f}
     print(f"[bench] side: tangent/forward cost {t_t / t_f:.2f} on Nr={q.Nr}", file=sys.stderr, flush=True)
+    S.close()
     return out
 
 
@@ -270,6 +271,7 @@ def main():
                           "batched_over_per_ik": value * t1}
     if tune is not None:
         line["autotune"] = tune
+    S.close()   # the side measurements build their own device tables
     if args.side_rmax > 0 and args.rmax == 0:
         line["side"] = side_measurements(args, p, ks_all, B)
     if not args.no_cpu:

@@ -166,19 +166,28 @@ typedef struct {
  * energy i; the same k may appear twice with two directions. */
 int  sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len,
                            sehf_scan_out* out, sehf_scan_tangent_out* tout);
-/* Autotuner of the batch width and the number of concurrent groups.  For
- * each candidate (batch in 1, 2, 4, ... 128 with groups * batch <= nk,
- * groups in 1..2) that fits in device memory, the batch buffers are
- * allocated and `probe_steps` radial steps of the forward recursion (Schur
- * inverses and ratio steps at full size, with the tangent when `tangent` is
- * set) are timed; a candidate list stops growing once the rate gains less
- * than 3%.  The fastest candidate in energy-steps per second is returned and
- * the handle's group count is set to it.  report[3*i .. 3*i+2] = (batch,
- * groups, energy-steps/s) of each probe (at most 32); NULL allowed.  The
- * batch buffers are released on return.  Replaces no reference interface:
- * the reference runs one energy at a time. */
+/* Autotuner of the batch width, the number of concurrent groups and the
+ * GEMM tile policy.  For each candidate (batch in 1, 2, 4, ... 128 with
+ * groups * batch <= nk, groups in 1..2) that fits in device memory, the batch
+ * buffers are allocated and `probe_steps` radial steps of the forward
+ * recursion (Schur inverses and ratio steps at full size, with the tangent
+ * when `tangent` is set) are timed, all groups starting together; a
+ * candidate list stops growing once the rate gains less than 3%.  At the
+ * fastest batch width the other GEMM tile policies (sehf_set_gemm_tiles) are
+ * probed as well.  The fastest combination in energy-steps per second is
+ * returned; the handle's group count and the process-wide tile policy are
+ * set to it.  report[4*i .. 4*i+3] = (batch, groups, tile policy,
+ * energy-steps/s) of each probe (at most 32); NULL allowed.  The batch
+ * buffers are released on return.  Replaces no reference interface: the
+ * reference runs one energy at a time. */
 int  sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, int tangent, int* batch_out,
-                        int* groups_out, double* report, int* n_report);
+                        int* groups_out, int* tiles_out, double* report, int* n_report);
+/* Process-wide tile policy of the FP64 GEMM: 0 (default) by shape, 128 x 128
+ * tiles over 8 waves where M > 64 and K >= 128, else 64 x 64 over 4 waves;
+ * 1 = 64 x 64 everywhere; 2 = 128 x 128 wherever M > 64.  Results do not
+ * depend on it beyond rounding order within a tile (they are bitwise equal
+ * across policies only when no product changes tile). */
+int  sehf_set_gemm_tiles(int policy);
 /* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
  * fit + dipole, per-batch host tables. */
 int  sehf_scan_stats(const sehf_scan* s, double* t);

@@ -9,10 +9,16 @@
 // each shape's rate.  Prints one line per shape with its HIP-event rate.
 //
 //   gemm_probe [batch=32] [reps=3] [calib_mib=1024]
+//   gemm_probe gj [batch=32] [reps=3]   -- the batched Gauss-Jordan inverse of
+//                                          orders 1566 and 961 instead
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "sehf_common.h"
@@ -20,8 +26,65 @@
 
 using namespace sehf;
 
+// Batched Gauss-Jordan inverse of random matrices (uniform entries, diagonal
+// raised by 2: generic pivoting, condition ~ n), timed with HIP events; a
+// sample of |A X - I| of the first matrix is printed as a check.
+static int gj_main(int batch, int reps) {
+    for (int n : {1566, 961}) {
+        const size_t per = static_cast<size_t>(n) * n;
+        std::vector<double> h(per * batch);
+        uint64_t st = 12345;
+        for (size_t i = 0; i < h.size(); ++i) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            h[i] = static_cast<double>(st >> 11) / 9007199254740992.0 - 0.5;
+        }
+        for (int b = 0; b < batch; ++b)
+            for (int i = 0; i < n; ++i) h[b * per + i + static_cast<size_t>(i) * n] += 2.0;
+        DevArray<double> A(per * batch), W(per * batch), X(per * batch);
+        SEHF_HIP_CHECK(hipMemcpy(A.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        GjBuffers gb;
+        gb.ensure(n, batch);
+        hipEvent_t e0, e1;
+        SEHF_HIP_CHECK(hipEventCreate(&e0));
+        SEHF_HIP_CHECK(hipEventCreate(&e1));
+        float total = 0.0f;
+        for (int r = 0; r <= reps; ++r) {   // r = 0 warms up
+            SEHF_HIP_CHECK(hipMemcpy(W.p, A.p, per * batch * sizeof(double), hipMemcpyDeviceToDevice));
+            SEHF_HIP_CHECK(hipEventRecord(e0, nullptr));
+            gj_inverse(W.p, X.p, n, n, static_cast<int64_t>(per), batch, false, gb.work(), nullptr);
+            SEHF_HIP_CHECK(hipEventRecord(e1, nullptr));
+            SEHF_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            SEHF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0) total += ms;
+        }
+        std::vector<double> x(per);
+        SEHF_HIP_CHECK(hipMemcpy(x.data(), X.p, per * sizeof(double), hipMemcpyDeviceToHost));
+        double worst = 0.0;
+        for (int j = 0; j < n; j += 7)
+            for (int i = 0; i < n; i += 5) {
+                double acc = 0.0;
+                for (int q = 0; q < n; ++q) acc += h[i + static_cast<size_t>(q) * n] * x[q + static_cast<size_t>(j) * n];
+                worst = std::max(worst, std::fabs(acc - (i == j ? 1.0 : 0.0)));
+            }
+        const double ms = total / reps;
+        std::printf("Gauss-Jordan inverse n=%d batch %d: %.3f ms  %.1f TFLOP/s (2n^3)  max|AX-I| sampled %.1e\n", n,
+                    batch, ms, 2.0 * n * static_cast<double>(n) * n * batch / (ms * 1e-3) / 1e12, worst);
+        std::fflush(stdout);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
+        if (argc > 1 && std::string(argv[1]) == "gj") {
+            const int batch = argc > 2 ? std::atoi(argv[2]) : 32;
+            const int reps = argc > 3 ? std::atoi(argv[3]) : 3;
+            if (batch < 1 || batch > 64 || reps < 1) throw std::runtime_error("usage: gemm_probe gj [batch<=64] [reps]");
+            return gj_main(batch, reps);
+        }
         const int batch = argc > 1 ? std::atoi(argv[1]) : 32;
         const int reps = argc > 2 ? std::atoi(argv[2]) : 3;
         const size_t calib = static_cast<size_t>(argc > 3 ? std::atoi(argv[3]) : 1024) << 20;

@@ -29,6 +29,12 @@ struct Gemm {
 };
 void gemm(const Gemm& g, hipStream_t s);
 
+// GEMM tile policy (process-wide): 0 = by shape (128 x 128 tiles over 8 waves
+// where M > 64 and K >= 128, else 64 x 64 over 4 waves), 1 = 64 x 64 for
+// every product, 2 = 128 x 128 wherever M > 64.  The autotuner probes them.
+void set_gemm_tiles(int policy);
+int gemm_tiles();
+
 // In-place batched inverse by blocked Gauss-Jordan elimination with partial
 // pivoting.  `mat` holds `batch` column-major n x n matrices (leading dim ld,
 // element stride `stride`).  The inverse is written to `out` (may NOT alias

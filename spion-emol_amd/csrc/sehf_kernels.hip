@@ -248,6 +248,15 @@ struct ProfScope {
     }
 };
 
+namespace {
+std::atomic<int> g_tiles{0};
+}
+void set_gemm_tiles(int policy) {
+    if (policy < 0 || policy > 2) throw std::runtime_error("GEMM tile policy must be 0, 1 or 2");
+    g_tiles.store(policy);
+}
+int gemm_tiles() { return g_tiles.load(); }
+
 void gemm(const Gemm& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
     GemmK p{g.M, g.N, g.K, g.A, g.lda, g.sA, g.tA, g.B, g.ldb, g.sB, g.tB, g.C, g.ldc, g.sC,
@@ -259,7 +268,9 @@ void gemm(const Gemm& g, hipStream_t s) {
     // products that fill it (45-51 TFLOP/s where 64 x 64 gives 36-42);
     // 64 x 64 over 4 waves for thin or shallow ones, which are bound by
     // latency and memory and want the larger grid.
-    if (g.M > 64 && g.K >= 128) {
+    const int pol = g_tiles.load(std::memory_order_relaxed);
+    const bool big = g.M > 64 && (pol == 2 || (pol == 0 && g.K >= 128));
+    if (big) {
         dim3 grid((g.M + 127) / 128, (g.N + 127) / 128, g.batch);
         hipLaunchKernelGGL((k_dgemm<128, 128, 2, 4>), grid, dim3(512), 0, s, p);
     } else {

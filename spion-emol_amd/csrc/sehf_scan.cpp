@@ -1224,7 +1224,7 @@ int sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int 
 }
 
 int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, int tangent, int* batch_out,
-                       int* groups_out, double* report, int* n_report) {
+                       int* groups_out, int* tiles_out, double* report, int* n_report) {
     return guarded([&] {
         if (nk <= 0 || !k) throw std::runtime_error("sehf_scan_autotune: empty k list");
         for (int i = 0; i < nk; ++i)
@@ -1237,71 +1237,96 @@ int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, i
         std::vector<double> dir(tangent ? 2 * static_cast<size_t>(nk) : 0);
         for (int i = 0; i < nk && tangent; ++i) { dir[2 * i] = 1.0; dir[2 * i + 1] = 0.0; }
         const int cand[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+        // one probe: allocate G groups of batch B, time `steps` forward steps
+        // of all groups started together; < 0 when the buffers do not fit
+        auto probe = [&](int B, int G) -> double {
+            s->groups.clear();   // release every batch buffer before sizing the candidate
+            try {
+                while (static_cast<int>(s->groups.size()) < G) {
+                    s->groups.emplace_back(new Group());
+                    s->groups.back()->init();
+                }
+                for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, tangent != 0);
+            } catch (const std::exception&) {
+                (void)hipGetLastError();
+                return -1.0;
+            }
+            // energies of the probe: group gi takes k[(gi*B + e) % nk]
+            std::vector<std::vector<double>> kg(G, std::vector<double>(B)), dg(G, std::vector<double>(2 * B));
+            for (int gi = 0; gi < G; ++gi)
+                for (int e = 0; e < B; ++e) {
+                    const int src = (gi * B + e) % nk;
+                    kg[gi][e] = k[src];
+                    if (tangent) { dg[gi][2 * e] = dir[2 * src]; dg[gi][2 * e + 1] = dir[2 * src + 1]; }
+                }
+            double wall_s = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {   // rep 0 warms the streams and kernels
+                std::vector<std::string> errs(G);
+                ProbeSync sync;
+                sync.total = G;
+                auto drive = [&](int gi) {
+                    try {
+                        SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
+                        run_batch(s, *s->groups[gi], kg[gi].data(), tangent ? dg[gi].data() : nullptr, B, L, nullptr,
+                                  nullptr, 0, rep == 0 ? 2 : steps, &sync);
+                    } catch (const std::exception& e) {
+                        errs[gi] = e.what();
+                        sync.arrive(wall());   // never leave the other groups waiting (no-op once all arrived)
+                    }
+                };
+                std::vector<std::thread> th;
+                for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
+                for (auto& x : th) x.join();
+                for (int gi = 0; gi < G; ++gi)
+                    if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
+                double tp = 0.0;
+                for (int gi = 0; gi < G; ++gi) tp = std::max(tp, s->groups[gi]->t_probe);
+                wall_s = std::max(tp, 1e-9);
+            }
+            return static_cast<double>(G) * B * steps / wall_s;   // energy-steps per second
+        };
         double best = -1.0;
-        int bestB = 1, bestG = 1, nrep = 0;
+        int bestB = 1, bestG = 1, bestT = gemm_tiles(), nrep = 0;
+        auto note = [&](int B, int G, int T, double rate) {
+            if (report && nrep < 32) {
+                report[4 * nrep] = B; report[4 * nrep + 1] = G; report[4 * nrep + 2] = T; report[4 * nrep + 3] = rate;
+            }
+            ++nrep;
+        };
+        // batch width and groups at the current tile policy
         for (int G = 1; G <= 2; ++G) {
             double prev = -1.0;
             for (int B : cand) {
                 if (G * B > nk) break;
-                s->groups.clear();   // release every batch buffer before sizing the next candidate
-                bool fits = true;
-                try {
-                    while (static_cast<int>(s->groups.size()) < G) {
-                        s->groups.emplace_back(new Group());
-                        s->groups.back()->init();
-                    }
-                    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, tangent != 0);
-                } catch (const std::exception&) {
-                    fits = false;
-                    (void)hipGetLastError();
-                }
-                if (!fits) break;
-                // energies of the probe: group gi takes k[(gi*B + e) % nk]
-                std::vector<std::vector<double>> kg(G, std::vector<double>(B)), dg(G, std::vector<double>(2 * B));
-                for (int gi = 0; gi < G; ++gi)
-                    for (int e = 0; e < B; ++e) {
-                        const int src = (gi * B + e) % nk;
-                        kg[gi][e] = k[src];
-                        if (tangent) { dg[gi][2 * e] = dir[2 * src]; dg[gi][2 * e + 1] = dir[2 * src + 1]; }
-                    }
-                double wall_s = 0.0;
-                for (int rep = 0; rep < 2; ++rep) {   // rep 0 warms the streams and kernels
-                    std::vector<std::string> errs(G);
-                    ProbeSync sync;
-                    sync.total = G;
-                    auto drive = [&](int gi) {
-                        try {
-                            SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
-                            run_batch(s, *s->groups[gi], kg[gi].data(), tangent ? dg[gi].data() : nullptr, B, L, nullptr,
-                                      nullptr, 0, rep == 0 ? 2 : steps, &sync);
-                        } catch (const std::exception& e) {
-                            errs[gi] = e.what();
-                            sync.arrive(wall());   // never leave the other groups waiting (no-op once all arrived)
-                        }
-                    };
-                    std::vector<std::thread> th;
-                    for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
-                    for (auto& x : th) x.join();
-                    for (int gi = 0; gi < G; ++gi)
-                        if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
-                    double tp = 0.0;
-                    for (int gi = 0; gi < G; ++gi) tp = std::max(tp, s->groups[gi]->t_probe);
-                    wall_s = std::max(tp, 1e-9);
-                }
-                const double rate = static_cast<double>(G) * B * steps / wall_s;   // energy-steps per second
-                if (report && nrep < 32) { report[3 * nrep] = B; report[3 * nrep + 1] = G; report[3 * nrep + 2] = rate; }
-                ++nrep;
+                const double rate = probe(B, G);
+                if (rate < 0.0) break;   // does not fit: larger batches do not either
+                note(B, G, bestT, rate);
                 if (rate > best) { best = rate; bestB = B; bestG = G; }
                 if (prev > 0.0 && rate < 1.03 * prev) break;   // saturated: larger batches only cost memory
                 prev = rate;
             }
         }
+        // GEMM tile policy at that batch width
+        const int T0 = bestT;
+        for (int T = 0; T <= 2; ++T) {
+            if (T == T0) continue;
+            set_gemm_tiles(T);
+            const double rate = probe(bestB, bestG);
+            note(bestB, bestG, T, rate);
+            if (rate > best) { best = rate; bestT = T; }
+        }
+        set_gemm_tiles(bestT);
         s->groups.clear();
         s->n_groups = bestG;
         if (batch_out) *batch_out = bestB;
         if (groups_out) *groups_out = bestG;
+        if (tiles_out) *tiles_out = bestT;
         if (n_report) *n_report = std::min(nrep, 32);
     });
+}
+
+int sehf_set_gemm_tiles(int policy) {
+    return guarded([&] { set_gemm_tiles(policy); });
 }
 
 int sehf_scan_stats(const sehf_scan* s, double* t) {

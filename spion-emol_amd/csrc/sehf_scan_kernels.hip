@@ -173,23 +173,26 @@ __device__ inline void amax_pos(double* addr, double v) {
 // Residual pass (AsymptoticAmplitudes.cpp:284-305): out[b*2+0] = max |fit - psi|
 // over open rows, out[b*2+1] = max |psi| over the nfit window points given
 // (points at stride Np^2, energies at stride sWin); the maxima accumulate
-// over calls.
+// over calls.  Each thread owns one row mu and RES_COLS columns, so a wave
+// issues one atomic per quantity.
+constexpr int RES_COLS = 64;
 __global__ void k_fit_resid(const double* win, const double* A, const double* B, const double* Jc,
                             const double* Yc, const double* coef, int Np, int nfit, int64_t sWin,
                             int64_t sC, double* out) {
     const int64_t b = blockIdx.z;
-    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x, j0 = blockIdx.y * RES_COLS;
     double r = 0.0, pm = 0.0;
     if (mu < Np) {
-        const int64_t s = (int64_t)Np * Np, o = mu + (int64_t)j * Np;
+        const int64_t s = (int64_t)Np * Np;
         const bool closed = coef[b * 5 * Np + 4 * Np + mu] != 0.0;
-        const double a = A[b * s + o], bb = B[b * s + o];
+        const int j1 = min(j0 + RES_COLS, Np);
         for (int i = 0; i < nfit; ++i) {
-            const double p = win[b * sWin + (int64_t)i * s + o];
-            pm = fmax(pm, fabs(p));
-            if (!closed) {
-                const double f = Jc[b * sC + (int64_t)i * Np + mu] * a + Yc[b * sC + (int64_t)i * Np + mu] * bb;
-                r = fmax(r, fabs(f - p));
+            const double jc = Jc[b * sC + (int64_t)i * Np + mu], yc = Yc[b * sC + (int64_t)i * Np + mu];
+            for (int j = j0; j < j1; ++j) {
+                const int64_t o = mu + (int64_t)j * Np;
+                const double p = win[b * sWin + (int64_t)i * s + o];
+                pm = fmax(pm, fabs(p));
+                if (!closed) r = fmax(r, fabs(jc * A[b * s + o] + yc * B[b * s + o] - p));
             }
         }
     }
@@ -202,7 +205,7 @@ __global__ void k_fit_resid(const double* win, const double* A, const double* B,
 void fit_resid(const double* win, const double* A, const double* B, const double* Jc, const double* Yc,
                const double* coef, int Np, int nfit, int64_t sWin, int64_t sC, double* out, int batch,
                hipStream_t s) {
-    dim3 g((Np + 255) / 256, Np, batch);
+    dim3 g((Np + 255) / 256, (Np + RES_COLS - 1) / RES_COLS, batch);
     hipLaunchKernelGGL(k_fit_resid, g, dim3(256), 0, s, win, A, B, Jc, Yc, coef, Np, nfit, sWin, sC, out);
 }
 

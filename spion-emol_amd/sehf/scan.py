@@ -135,28 +135,34 @@ class Scan:
         return res
 
     def autotune(self, ks, probe_steps: int = 8, tangent: bool = False) -> dict:
-        """Pick the batch width and group count for these energies by timing
-        probe_steps radial steps per candidate (sehf_scan_autotune); the
-        handle keeps the chosen group count."""
+        """Pick the batch width, group count and GEMM tile policy for these
+        energies by timing probe_steps radial steps per candidate
+        (sehf_scan_autotune); the handle keeps the chosen group count and the
+        process the chosen tile policy."""
         L = load()
         ks = np.ascontiguousarray(ks, dtype=np.float64)
-        b, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        rep = (ctypes.c_double * 96)()
+        b, g, t, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rep = (ctypes.c_double * 128)()
         check(L.sehf_scan_autotune(self._h, ks.ctypes.data_as(_dp), len(ks), int(probe_steps), 1 if tangent else 0,
-                                   ctypes.byref(b), ctypes.byref(g), rep, ctypes.byref(n)), "sehf_scan_autotune")
-        table = [{"batch": int(rep[3 * i]), "groups": int(rep[3 * i + 1]), "energy_steps_per_s": rep[3 * i + 2]}
-                 for i in range(n.value)]
-        return {"batch": b.value, "groups": g.value, "probes": table}
+                                   ctypes.byref(b), ctypes.byref(g), ctypes.byref(t), rep, ctypes.byref(n)),
+              "sehf_scan_autotune")
+        table = [{"batch": int(rep[4 * i]), "groups": int(rep[4 * i + 1]), "tiles": int(rep[4 * i + 2]),
+                  "energy_steps_per_s": rep[4 * i + 3]} for i in range(n.value)]
+        return {"batch": b.value, "groups": g.value, "tiles": t.value, "probes": table}
 
     def stats(self) -> dict:
         t = (ctypes.c_double * 8)()
         check(load().sehf_scan_stats(self._h, t))
         return dict(zip(("setup", "forward", "kmatrix", "backprop", "fit_dipole", "host_tables"), list(t)[:6]))
 
-    def __del__(self):
+    def close(self):
+        """Release the device tables and batch buffers (sehf_scan_destroy)."""
         if getattr(self, "_h", None):
             load().sehf_scan_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def solve_scan(problem, ks, batch: int = 0, seg_len: int = 0, ctx=None):

@@ -136,10 +136,21 @@ def test_autotune_then_run(ctx):
     assert 1 <= tune["groups"] <= 2 and 1 <= tune["batch"] <= len(ks)
     assert tune["batch"] * tune["groups"] <= len(ks)
     best = max(tune["probes"], key=lambda r: r["energy_steps_per_s"])
-    assert (best["batch"], best["groups"]) == (tune["batch"], tune["groups"])
+    assert (best["batch"], best["groups"], best["tiles"]) == (tune["batch"], tune["groups"], tune["tiles"])
+    assert {r["tiles"] for r in tune["probes"]} == {0, 1, 2}
+    # the result with the chosen tiles against the reference rounding floor
+    import oracle_py as O
     got = S.run(ks, batch=tune["batch"])
-    for a, b in zip(ref, got):
+    ref_o = O.solve_scan(p, ks[:2])
+    from conftest import rel_err
+    for e in range(2):
+        assert rel_err(got[e]["D"], ref_o[e]["D"]) < 1e-9
+    from sehf._lib import check, load
+    check(load().sehf_set_gemm_tiles(0))
+    back = S.run(ks, batch=8)
+    for a, b in zip(ref, back):
         for key in ("K", "A", "B", "D"):
             assert np.array_equal(a[key], b[key]), (key, a["k"])
     tt = S.autotune(ks[:4], probe_steps=4, tangent=True)
     assert tt["probes"] and tt["batch"] * tt["groups"] <= 4
+    check(load().sehf_set_gemm_tiles(0))
