@@ -124,7 +124,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="energies per batch; 0 = batch width, groups and GEMM tiles chosen by the library's "
+                         "autotuner (timed probes of radial steps, outside the timed region)")
     ap.add_argument("--groups", type=int, default=1,
                     help="concurrent energy groups (streams); a step is groups x batch energies")
     ap.add_argument("--config", default="h2o_l30")
@@ -134,9 +136,7 @@ def main():
     ap.add_argument("--prof-every", type=int, default=97)
     ap.add_argument("--per-ik", action="store_true",
                     help="also time the plain per-ik path (batch 1, one group) on one energy of the full grid")
-    ap.add_argument("--autotune", action="store_true",
-                    help="choose batch width and group count with the library's autotuner (probe of radial steps) "
-                         "instead of --batch / --groups")
+
     ap.add_argument("--tangent", action="store_true",
                     help="time the batched scan with its forward-mode derivative along E (batched + adjoint)")
     ap.add_argument("--side-rmax", type=float, default=5.0,
@@ -176,7 +176,7 @@ def main():
     B = args.batch
     G = max(1, args.groups)
     tune = None
-    if args.autotune:
+    if B <= 0:
         t0 = time.time()
         tune = S.autotune(ks_all[(np.arange(min(128, len(ks_all))) * 37) % len(ks_all)], probe_steps=8,
                           tangent=args.tangent)

@@ -154,3 +154,19 @@ def test_autotune_then_run(ctx):
     tt = S.autotune(ks[:4], probe_steps=4, tangent=True)
     assert tt["probes"] and tt["batch"] * tt["groups"] <= 4
     check(load().sehf_set_gemm_tiles(0))
+
+
+def test_scan_h2o_l8_midsize(ctx):
+    """A mid-size system (l_cont = 8, three occupied orbitals: N_psi = 81,
+    N_f = 243, N_tot = 324, 501 points) whose order-324 inverses and
+    back-propagation products run on the 128 x 128 GEMM tiles, against the
+    oracle with its measured rounding floor."""
+    import oracle_py as O
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=8, dr=0.02, r_max=10.0, n_occ=3, seed=1, l_orb_sce=6, allow_closed=True)
+    assert p.n_tot == 324
+    ks = np.array([0.55, 1.05])
+    got = scan.solve_scan(p, ks, ctx=ctx)
+    ref = O.solve_scan(p, ks)
+    floor = O.noise_floor(p, ks)
+    _compare(p, ks, got, ref, label="h2o_l8", floor_list=floor)

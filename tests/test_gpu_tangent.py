@@ -134,3 +134,16 @@ def test_tangent_h2_l8_full_energy(ctx):
     for e, k in enumerate(ks):
         ref, bound = _oracle_with_floor(O, p, float(k), (1.0, 0.0))
         _check(got[e], ref, bound, f"h2_l8 k={k:.2f}")
+
+
+def test_tangent_h2o_l8_midsize(ctx):
+    """The derivative along E and alpha on the mid-size system (N_tot = 324:
+    128 x 128 GEMM tiles in the ratio-matrix tangents and the
+    back-propagation products), one energy, mixed direction."""
+    import oracle_py as O
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=8, dr=0.02, r_max=10.0, n_occ=3, seed=1, l_orb_sce=6, pol=(9.8, 1.5),
+                       allow_closed=True)
+    got = scan.Scan(p, ctx).run(np.array([0.85]), tangent=(1.0, 0.25))
+    ref, bound = _oracle_with_floor(O, p, 0.85, (1.0, 0.25))
+    _check(got[0], ref, bound, "h2o_l8 k=0.85")
