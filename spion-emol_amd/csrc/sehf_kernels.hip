@@ -2,8 +2,8 @@
 //
 //  * k_dgemm       : batched FP64 GEMM on v_mfma_f64_16x16x4f64, 128x128 output
 //                    tile per 8-wave workgroup for deep products, 64x64 per
-//                    4 waves for thin ones; K staged through LDS in slabs of
-//                    16; XCD-aware tile order.
+//                    4 waves for shallow ones, 16x128 / 32x128 for thin ones;
+//                    K staged through LDS in slabs of 16; XCD-aware tile order.
 //  * Gauss-Jordan  : in-place inversion with partial pivoting, recursive over
 //                    column blocks.  Leaves (<=16 columns) are factored by one
 //                    1024-thread workgroup per matrix with the panel held in
@@ -266,13 +266,21 @@ void gemm(const Gemm& g, hipStream_t s) {
     // Tile by shape (measured on the Gauss-Jordan updates and back-propagation
     // products at batch 32): 128 x 128 over 8 waves (64 x 32 each) for deep
     // products that fill it (45-51 TFLOP/s where 64 x 64 gives 36-42);
-    // 64 x 64 over 4 waves for thin or shallow ones, which are bound by
-    // latency and memory and want the larger grid.
+    // 64 x 64 over 4 waves for shallow ones, which are bound by latency and
+    // memory and want the larger grid; 16- and 32-row tiles for the thin
+    // block updates of the innermost Gauss-Jordan levels (M = 16 or 32
+    // columns), 20-30% faster there than 64-row tiles that are mostly empty.
     const int pol = g_tiles.load(std::memory_order_relaxed);
     const bool big = g.M > 64 && (pol == 2 || (pol == 0 && g.K >= 128));
     if (big) {
         dim3 grid((g.M + 127) / 128, (g.N + 127) / 128, g.batch);
         hipLaunchKernelGGL((k_dgemm<128, 128, 2, 4>), grid, dim3(512), 0, s, p);
+    } else if (pol != 1 && g.M <= 16) {
+        dim3 grid((g.M + 15) / 16, (g.N + 127) / 128, g.batch);
+        hipLaunchKernelGGL((k_dgemm<16, 128, 1, 4>), grid, dim3(256), 0, s, p);
+    } else if (pol != 1 && g.M <= 32) {
+        dim3 grid((g.M + 31) / 32, (g.N + 127) / 128, g.batch);
+        hipLaunchKernelGGL((k_dgemm<32, 128, 1, 4>), grid, dim3(256), 0, s, p);
     } else {
         dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, g.batch);
         hipLaunchKernelGGL((k_dgemm<64, 64, 2, 2>), grid, dim3(256), 0, s, p);
