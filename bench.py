@@ -28,6 +28,14 @@ sys.path.insert(0, os.path.join(ROOT, "spion-emol_amd"))
 
 PEAK_FP64_MFMA_TFLOPS = 78.6      # MI355X dense FP64 matrix peak (datasheet)
 
+# HBM bytes of one k_dgemm launch of the top-level Gauss-Jordan update of the
+# order-1566 inverse (783 x 1566 x 783, the largest single share of GEMM time
+# at l_cont = 30), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# spion-emol_amd/_build/gemm_probe and calibrated by a 1 GiB device copy
+# (profiles/k_dgemm_hbm_traffic_gemm_probe_b32.md).  batch -> (measured bytes,
+# algorithmic bytes).  Counters cannot be collected inside the scan itself.
+TOP_UPDATE_TRAFFIC = {32: (1.7831e+09, 1.0987e+09), 48: (2.7581e+09, 1.6480e+09)}
+
 
 def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
     """Algorithmic FP64 flop of one energy point (the path's flop model):
@@ -238,6 +246,12 @@ def main():
             if not np.all(np.isfinite(r["D"])) or (args.tangent and not np.all(np.isfinite(r["dD"]))):
                 raise SystemExit("non-finite dipole in bench output")
 
+    traffic, traffic_note = None, None
+    if args.config == "h2o_l30" and args.rmax == 0 and B in TOP_UPDATE_TRAFFIC:
+        tb, ta = TOP_UPDATE_TRAFFIC[B]
+        traffic = tb
+        traffic_note = (f"HBM bytes of the 783x1566x783 Gauss-Jordan update launch at batch {B} "
+                        f"(rocprofv3 --pmc on gemm_probe, calibrated); algorithmic {ta:.3e} B, ratio {tb / ta:.2f}")
     line = {
         "metric": "energy points/sec @l_cont=30 (batched vs per-ik, 1 GPU); FP64-MFMA %; adjoint/fwd cost",
         "value": value, "unit": "energy points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -251,7 +265,8 @@ def main():
                    "batch_energies": B, "groups": G, "energies_per_step": E_step, "tangent": bool(args.tangent),
                    "seg_len": args.seg_len or "ceil(sqrt(Nr))"},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
+                     "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
+                     "traffic_launch": traffic_note,
                      "kernel": "k_dgemm", "sampled_launches": int(prof[2]),
                      "avg_launch_us": avg_launch_us, "event_pair_overhead_us": prof[15] * 1e3},
         "model_tflop_per_energy": mflop / 1e12,

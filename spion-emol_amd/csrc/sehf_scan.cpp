@@ -1162,49 +1162,49 @@ namespace {
 
 void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len, sehf_scan_out* out,
               sehf_scan_tangent_out* tout) {
-        if (nk <= 0) return;
-        if (!out) throw std::runtime_error("sehf_scan_run: out must not be NULL");
-        for (int i = 0; i < nk; ++i)
-            if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_run: every k must be > 0");
-        if (dir)
-            for (int i = 0; i < 2 * nk; ++i)
-                if (!std::isfinite(dir[i])) throw std::runtime_error("sehf_scan_run_tangent: directions must be finite");
-        const int N = s->Nr - 1;
-        int L = seg_len > 0 ? seg_len : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
-        L = std::max(L, s->n_start + 1);
-        const int n_seg = (N + L - 1) / L;
-        const int n_fit = 200;
-        int B = batch > 0 ? std::min(batch, nk) : nk;
-        const int n_batches = (nk + B - 1) / B;
-        const int G = std::max(1, std::min(s->n_groups, n_batches));
-        while (static_cast<int>(s->groups.size()) < G) {
-            s->groups.emplace_back(new Group());
-            s->groups.back()->init();
-        }
-        for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, dir != nullptr);
-        if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
-        // batch i runs on group i % G; each group is driven by its own host thread
-        std::vector<std::string> errs(G);
-        auto drive = [&](int gi) {
-            try {
-                SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
-                for (int bi = gi; bi < n_batches; bi += G) {
-                    const int e0 = bi * B, nb = std::min(B, nk - e0);
-                    run_batch(s, *s->groups[gi], k + e0, dir ? dir + 2 * e0 : nullptr, nb, L, out, tout, e0);
-                }
-            } catch (const std::exception& e) {
-                errs[gi] = e.what();
+    if (nk <= 0) return;
+    if (!out) throw std::runtime_error("sehf_scan_run: out must not be NULL");
+    for (int i = 0; i < nk; ++i)
+        if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_run: every k must be > 0");
+    if (dir)
+        for (int i = 0; i < 2 * nk; ++i)
+            if (!std::isfinite(dir[i])) throw std::runtime_error("sehf_scan_run_tangent: directions must be finite");
+    const int N = s->Nr - 1;
+    int L = seg_len > 0 ? seg_len : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
+    L = std::max(L, s->n_start + 1);
+    const int n_seg = (N + L - 1) / L;
+    const int n_fit = 200;
+    int B = batch > 0 ? std::min(batch, nk) : nk;
+    const int n_batches = (nk + B - 1) / B;
+    const int G = std::max(1, std::min(s->n_groups, n_batches));
+    while (static_cast<int>(s->groups.size()) < G) {
+        s->groups.emplace_back(new Group());
+        s->groups.back()->init();
+    }
+    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, dir != nullptr);
+    if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
+    // batch i runs on group i % G; each group is driven by its own host thread
+    std::vector<std::string> errs(G);
+    auto drive = [&](int gi) {
+        try {
+            SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
+            for (int bi = gi; bi < n_batches; bi += G) {
+                const int e0 = bi * B, nb = std::min(B, nk - e0);
+                run_batch(s, *s->groups[gi], k + e0, dir ? dir + 2 * e0 : nullptr, nb, L, out, tout, e0);
             }
-        };
-        if (G == 1) {
-            drive(0);
-        } else {
-            std::vector<std::thread> th;
-            for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
-            for (auto& x : th) x.join();
+        } catch (const std::exception& e) {
+            errs[gi] = e.what();
         }
-        for (int gi = 0; gi < G; ++gi)
-            if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
+    };
+    if (G == 1) {
+        drive(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int gi = 0; gi < G; ++gi) th.emplace_back(drive, gi);
+        for (auto& x : th) x.join();
+    }
+    for (int gi = 0; gi < G; ++gi)
+        if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
 }
 
 }  // namespace

@@ -181,7 +181,9 @@ def beta(D_x, D_y, D_z, mu_list=None, n_theta: int = 40, n_phi: int = 80):
 
 def from_scan(results, E_init: float, device=None):
     """Observables of a list of scan results (one dict per energy, as
-    returned by Scan.run) for the initial-state energy E_init (Hartree)."""
+    returned by Scan.run) for the initial-state energy E_init (Hartree).
+    Results of a tangent run along E (Scan.run(..., tangent=(1, 0))) also
+    give the delay from the analytic energy derivative (tau_*_analytic)."""
     dev = device or ("cuda" if torch.cuda.is_available() else "cpu")
     k = torch.tensor([r["k"] for r in results], dtype=torch.float64, device=dev)
     D = torch.as_tensor(np.stack([r["D"] for r in results]), dtype=torch.complex128, device=dev)  # [nE, 6, N]
@@ -195,4 +197,9 @@ def from_scan(results, E_init: float, device=None):
     if len(results) >= 2:
         out["tau_L"] = time_delay(k, DL)
         out["tau_V"] = time_delay(k, DV)
+    if all("dD" in r for r in results):
+        # results of a tangent run along E (t_E = 1): the analytic delay
+        dD = torch.as_tensor(np.stack([r["dD"] for r in results]), dtype=torch.complex128, device=dev)
+        out["tau_L_analytic"] = time_delay_analytic(DL, dD[:, 0:3, :].reshape(len(results), -1))
+        out["tau_V_analytic"] = time_delay_analytic(DV, dD[:, 3:6, :].reshape(len(results), -1))
     return out

@@ -34,3 +34,24 @@ def test_observables_h2o_l4():
         err = np.abs(g - w).max() / max(np.abs(w).max(), 1e-300)
         print(key, f"{err:.2e}")
         assert err < 1e-8, (key, err)
+
+
+def test_analytic_delay_from_tangent_run():
+    """from_scan on a tangent run along E gives the delay from the analytic
+    dd/dE; against the same formula on the oracle's tangent, within 1e-6 of
+    max(1, |tau|) (the derivative's parity bound carried through tau)."""
+    import oracle_py as O
+    from sehf import scan, synth, observables
+    from sehf.steppers import GpuContext
+    p = synth.h2o_like(l_cont=4, dr=0.02, r_max=14.0, n_occ=3, seed=1)
+    ks = np.array([0.6, 0.95, 1.25])
+    got = scan.Scan(p, GpuContext(True)).run(ks, batch=3, tangent=(1.0, 0.0))
+    obs = observables.from_scan(got, float(p.orb_energies[p.n_occ - 1]), device="cuda")
+    for e, k in enumerate(ks):
+        ref = O.solve(p, float(k), tangent=(1.0, 0.0))
+        for gauge, sl in (("L", slice(0, 3)), ("V", slice(3, 6))):
+            D, dD = ref["D"][sl].ravel(), ref["dD"][sl].ravel()
+            want = (np.conj(D) * dD).sum().imag / (np.abs(D) ** 2).sum()
+            g = float(obs[f"tau_{gauge}_analytic"][e])
+            print(gauge, k, g, want)
+            assert abs(g - want) < 1e-6 * max(1.0, abs(want)), (gauge, k, g, want)
