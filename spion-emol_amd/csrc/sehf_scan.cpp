@@ -34,6 +34,7 @@
 #include "sehf_host_math.h"
 #include "sehf_scan_kernels.h"
 #include "sehf_setup.h"
+#include "sehf_setup_kernels.h"
 
 namespace sehf {
 std::string& last_error_ref();
@@ -154,22 +155,79 @@ void build_shared(sehf_scan* s) {
     s->nstab = std::min(Setup::stab_n_max, Nr);
     s->A0h.assign(s->nstab, {});
     s->S0h.assign(s->nstab, {});
+    // The dense V(r_n) and B(r_n) are built on the device from V_s(r), chi(r)
+    // and the sparse Gaunt tables (sehf_setup_kernels.hip); with
+    // SEHF_TABLES_ON_HOST set they are built on the host and uploaded instead
+    // (the reference path of the equivalence test: the tables are bitwise
+    // equal).
+    const bool on_host = std::getenv("SEHF_TABLES_ON_HOST") != nullptr;
     const int chunk = 32;
-    std::vector<double> Vh(pp * chunk), Bh(std::max<size_t>(pf, 1) * chunk);
-    DevArray<double> Vd(pp * chunk), Bd(std::max<size_t>(pf, 1) * chunk), Bsc(std::max<size_t>(pf, 1) * chunk);
+    const int n_exp = S.n_exp, n_lam = S.n_lambda, n_occ = S.n_occ;
+    std::vector<double> Vh(on_host ? pp * chunk : pp), Bh(on_host ? std::max<size_t>(pf, 1) * chunk : 1);
+    DevArray<double> Vd(pp * chunk), Bd(std::max<size_t>(pf, 1) * chunk);
+    DevArray<int> d_pi, d_pj, d_vsig, d_lmu, d_xlam;
+    DevArray<int64_t> d_vptr, d_xptr;
+    DevArray<double> d_vval, d_xval, d_vs, d_vp, d_ir2, d_chi, d_fac;
+    DevArray<unsigned char> d_live;
+    std::vector<double> vs_h, vp_h(chunk), ir2_h(chunk), chi_h, fac_h(chunk);
+    std::vector<unsigned char> live_h(chunk);
+    auto upi = [&](auto& d, const auto& h) {
+        d.alloc(std::max<size_t>(h.size(), 1));
+        if (!h.empty())
+            SEHF_HIP_CHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, st));
+    };
+    if (!on_host) {
+        upi(d_pi, S.pot_pair_i()); upi(d_pj, S.pot_pair_j()); upi(d_vptr, S.pot_ptr());
+        upi(d_vsig, S.pot_sigma()); upi(d_vval, S.pot_val()); upi(d_lmu, S.l_mu);
+        upi(d_xptr, S.exch_ptr()); upi(d_xlam, S.exch_lambda()); upi(d_xval, S.exch_val());
+        vs_h.resize(static_cast<size_t>(chunk) * n_exp);
+        chi_h.resize(static_cast<size_t>(chunk) * n_occ * n_lam);
+        d_vs.alloc(vs_h.size()); d_vp.alloc(chunk); d_ir2.alloc(chunk); d_live.alloc(chunk);
+        d_chi.alloc(std::max<size_t>(chi_h.size(), 1)); d_fac.alloc(chunk);
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    const int64_t npairs = static_cast<int64_t>(S.pot_pair_i().size());
+    const double h2_12 = s->h * s->h / 12.0;
     for (int n0 = 0; n0 < Nr; n0 += chunk) {
         const int nc = std::min(chunk, Nr - n0);
-        S.build_chunk(n0, nc, Vh.data(), Bh.data());
-        for (int q = 0; q < nc; ++q)
-            for (int i = 0; i < Np; ++i)
-                cvd_h[static_cast<size_t>(n0 + q) * Np + i] = s->c6 * Vh[q * pp + i + static_cast<size_t>(i) * Np];
-        SEHF_HIP_CHECK(hipMemcpyAsync(Vd.p, Vh.data(), nc * pp * sizeof(double), hipMemcpyHostToDevice, st));
+        const int nb = std::max(0, std::min(nc, s->n_bd - n0));
+        if (on_host) {
+            S.build_chunk(n0, nc, Vh.data(), Bh.data());
+            for (int q = 0; q < nc; ++q)
+                for (int i = 0; i < Np; ++i)
+                    cvd_h[static_cast<size_t>(n0 + q) * Np + i] = s->c6 * Vh[q * pp + i + static_cast<size_t>(i) * Np];
+            SEHF_HIP_CHECK(hipMemcpyAsync(Vd.p, Vh.data(), nc * pp * sizeof(double), hipMemcpyHostToDevice, st));
+            if (Nf > 0 && nb > 0)
+                SEHF_HIP_CHECK(hipMemcpyAsync(Bd.p, Bh.data(), nb * pf * sizeof(double), hipMemcpyHostToDevice, st));
+        } else {
+            S.vsigma_table(n0, nc, vs_h.data());
+            for (int q = 0; q < nc; ++q) {
+                const double r = (n0 + q) * s->h;
+                live_h[q] = (r > 1e-14) ? 1 : 0;
+                vp_h[q] = S.polarization(r);
+                ir2_h[q] = 1.0 / (r * r);
+                fac_h[q] = (r < 1e-10) ? 0.0 : 2.0 * std::sqrt(2.0 * M_PI) / r;
+            }
+            SEHF_HIP_CHECK(hipMemcpyAsync(d_vs.p, vs_h.data(), static_cast<size_t>(nc) * n_exp * sizeof(double),
+                                          hipMemcpyHostToDevice, st));
+            SEHF_HIP_CHECK(hipMemcpyAsync(d_vp.p, vp_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
+            SEHF_HIP_CHECK(hipMemcpyAsync(d_ir2.p, ir2_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
+            SEHF_HIP_CHECK(hipMemcpyAsync(d_live.p, live_h.data(), nc, hipMemcpyHostToDevice, st));
+            SEHF_HIP_CHECK(hipMemcpyAsync(d_fac.p, fac_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
+            build_V(Vd.p, d_vs.p, n_exp, d_pi.p, d_pj.p, d_vptr.p, d_vsig.p, d_vval.p, npairs, Np, d_lmu.p, d_vp.p,
+                    d_ir2.p, d_live.p, nc, st);
+            scaled_diag(s->cvd.p + static_cast<size_t>(n0) * Np, Vd.p, s->c6, Np, nc, st);
+            if (Nf > 0 && nb > 0) {
+                S.chi_table(n0, nb, chi_h.data());
+                SEHF_HIP_CHECK(hipMemcpyAsync(d_chi.p, chi_h.data(), static_cast<size_t>(nb) * n_occ * n_lam * sizeof(double),
+                                              hipMemcpyHostToDevice, st));
+                build_B(Bd.p, d_chi.p, n_occ, n_lam, d_xptr.p, d_xlam.p, d_xval.p, Np, S.n_sigma, d_fac.p, h2_12, nb, st);
+            }
+        }
         double* S0c = s->S0.p + static_cast<size_t>(n0) * pp;
         // S0 = I - h^2/6 V
         axpbyi(S0c, Np, pp, Vd.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, nc, st);
-        const int nb = std::max(0, std::min(nc, s->n_bd - n0));
         if (Nf > 0 && nb > 0) {
-            SEHF_HIP_CHECK(hipMemcpyAsync(Bd.p, Bh.data(), nb * pf * sizeof(double), hipMemcpyHostToDevice, st));
             for (int q = 0; q < nb; ++q)
                 col_scale(s->BD.p + static_cast<size_t>(n0 + q) * pf, Np, Bd.p + q * pf, Np,
                           s->Dinv.p + static_cast<size_t>(n0 + q) * Nf, Np, Nf, st);
@@ -186,9 +244,14 @@ void build_shared(sehf_scan* s) {
         // host copies of the small-r matrices that may need the stability shift
         for (int q = 0; q < nc && n0 + q < s->nstab; ++q) {
             const int n = n0 + q;
+            const double* Vq = Vh.data() + (on_host ? q * pp : 0);
+            if (!on_host) {
+                SEHF_HIP_CHECK(hipMemcpyAsync(Vh.data(), Vd.p + q * pp, pp * sizeof(double), hipMemcpyDeviceToHost, st));
+                SEHF_HIP_CHECK(hipStreamSynchronize(st));
+            }
             auto& A0 = s->A0h[n];
             A0.resize(pp);
-            for (size_t t = 0; t < pp; ++t) A0[t] = -s->c6 * Vh[q * pp + t];
+            for (size_t t = 0; t < pp; ++t) A0[t] = -s->c6 * Vq[t];
             for (int i = 0; i < Np; ++i) A0[i + static_cast<size_t>(i) * Np] += 1.0;
             s->S0h[n].resize(pp);
             SEHF_HIP_CHECK(hipMemcpyAsync(s->S0h[n].data(), S0c + q * pp, pp * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -196,15 +259,19 @@ void build_shared(sehf_scan* s) {
         if (n0 + nc >= Nr) {   // boundary n = Nr-1: V_N and unscaled B_N
             const int q = Nr - 1 - n0;
             s->VN.alloc(pp);
-            SEHF_HIP_CHECK(hipMemcpyAsync(s->VN.p, Vh.data() + q * pp, pp * sizeof(double), hipMemcpyHostToDevice, st));
+            SEHF_HIP_CHECK(hipMemcpyAsync(s->VN.p, Vd.p + q * pp, pp * sizeof(double), hipMemcpyDeviceToDevice, st));
             if (Nf > 0) {
                 s->BN.alloc(pf);
-                SEHF_HIP_CHECK(hipMemcpyAsync(s->BN.p, Bh.data() + q * pf, pf * sizeof(double), hipMemcpyHostToDevice, st));
+                if (q < nb)
+                    SEHF_HIP_CHECK(hipMemcpyAsync(s->BN.p, Bd.p + q * pf, pf * sizeof(double), hipMemcpyDeviceToDevice, st));
+                else   // beyond the exchange cut-off B is zero (ExchangeCoupling.cpp:66-136)
+                    fill(s->BN.p, pf, 0.0, st);
             }
         }
         SEHF_HIP_CHECK(hipStreamSynchronize(st));
     }
-    SEHF_HIP_CHECK(hipMemcpyAsync(s->cvd.p, cvd_h.data(), cvd_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    if (on_host)
+        SEHF_HIP_CHECK(hipMemcpyAsync(s->cvd.p, cvd_h.data(), cvd_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     // Gershgorin bounds of A0_n and S0_n (SchurInverter.cpp:30-49)
     auto gersh = [Np](const std::vector<double>& M) {

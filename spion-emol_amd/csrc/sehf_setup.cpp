@@ -188,6 +188,52 @@ double Setup::polarization(double r) const {
     return 0.0;
 }
 
+void Setup::vsigma_table(int n0, int count, double* out) const {
+    // v_sigma(r) for many r: the spherical harmonics of the nuclei do not
+    // depend on r, so they are evaluated once; every element is formed with
+    // the same operations in the same order as v_sigma.
+    const int nmax = std::min(n_exp, P.nlm_vh);
+    std::vector<double> Yat(static_cast<size_t>(P.n_atoms) * n_exp, 0.0), Rat(P.n_atoms);
+    for (int a = 0; a < P.n_atoms; ++a) {
+        const double x = P.xyz[3 * a], y = P.xyz[3 * a + 1], z = P.xyz[3 * a + 2];
+        const double R = std::sqrt(x * x + y * y + z * z);
+        Rat[a] = R;
+        double th = 0.0, ph = 0.0;
+        if (R > 1e-14) { th = std::acos(std::clamp(z / R, -1.0, 1.0)); ph = std::atan2(y, x); }
+        for (int l = 0; l <= l_exp; ++l)
+            for (int m = -l; m <= l; ++m)
+                Yat[static_cast<size_t>(a) * n_exp + lm_index(l, m)] =
+                    (R < 1e-14) ? ((l == 0) ? 1.0 / (2.0 * std::sqrt(M_PI)) : 0.0) : real_ylm(l, m, th, ph);
+    }
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < count; ++q) {
+        const int ir = n0 + q;
+        const double r = ir * dr;
+        double* o = out + static_cast<size_t>(q) * n_exp;
+        for (int s = 0; s < n_exp; ++s) o[s] = 0.0;
+        for (int s = 0; s < nmax; ++s) o[s] += P.V_H[static_cast<size_t>(s) * Nr + ir];
+        for (int a = 0; a < P.n_atoms; ++a) {
+            const double Z = P.Z[a], R = Rat[a];
+            const double rl = std::min(r, R), rg = std::max(r, R);
+            for (int l = 0; l <= l_exp; ++l) {
+                const double radial = (4.0 * M_PI / (2.0 * l + 1.0)) * std::pow(rl, l) / std::pow(rg, l + 1);
+                for (int m = -l; m <= l; ++m) {
+                    const double Y = Yat[static_cast<size_t>(a) * n_exp + lm_index(l, m)];
+                    if (std::fabs(Y) > 1e-15) o[lm_index(l, m)] += -Z * radial * Y;
+                }
+            }
+        }
+    }
+}
+
+void Setup::chi_table(int n0, int count, double* out) const {
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < count; ++q)
+        for (int j = 0; j < n_occ; ++j)
+            for (int lam = 0; lam < n_lambda; ++lam)
+                out[(static_cast<size_t>(q) * n_occ + j) * n_lambda + lam] = chi(j, lam, n0 + q);
+}
+
 double Setup::polarization_per_alpha(int ir) const {
     const double r = ir * dr;
     if (!(r > 1e-14) || r < 1.0e-6) return 0.0;   // V(ir) is zero at r = 0

@@ -51,6 +51,21 @@ public:
     //   dcorr (6 x n_occ): int rfac phi_a . Xi_s
     void dipole_tables(std::vector<double>& rows, std::vector<double>& dcorr, int& n_pts) const;
 
+    // ---- inputs of the device-side table build (sehf_setup_kernels.hip):
+    // the same numbers potential() and exchange_B() combine on the host.
+    // V_s(r_n) for n in [n0, n0+count): out[(n-n0) * n_exp + s].
+    void vsigma_table(int n0, int count, double* out) const;
+    // chi_j,lambda(r_n) = r_n F_j,lambda(r_n): out[((n-n0) * n_occ + j) * n_lambda + lambda].
+    void chi_table(int n0, int count, double* out) const;
+    const std::vector<int>& pot_pair_i() const { return pair_i_; }
+    const std::vector<int>& pot_pair_j() const { return pair_j_; }
+    const std::vector<int64_t>& pot_ptr() const { return vptr_; }
+    const std::vector<int>& pot_sigma() const { return vsig_; }
+    const std::vector<double>& pot_val() const { return vval_; }
+    const std::vector<int64_t>& exch_ptr() const { return xptr_; }
+    const std::vector<int>& exch_lambda() const { return xlam_; }
+    const std::vector<double>& exch_val() const { return xval_; }
+
     static constexpr double W_min = 0.001;        // SchurInverter.hpp:74
     static constexpr int stab_n_max = 60;          // SchurInverter.hpp:75
     static constexpr double shift_margin = 0.01;   // SchurInverter.hpp:76
