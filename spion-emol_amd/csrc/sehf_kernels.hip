@@ -722,16 +722,23 @@ __global__ void k_copy_block(const double* mat, int64_t ld, int64_t stride, int 
 }
 
 // Column-interchange composition: perm = S_{n-1} ... S_0 applied to identity.
-__global__ void k_gj_perm(const int* piv, int* perm, int n) {
-    extern __shared__ int sp[];
+// With `staged` the interchange record is copied to LDS first, so the serial
+// chain of swaps waits on LDS rather than on a global load per step (orders
+// up to 8192, 64 KB of LDS).
+__global__ void k_gj_perm(const int* piv, int* perm, int n, int staged) {
+    extern __shared__ int sp[];   // [0, n): permutation, [n, 2n): interchange record when staged
     const int64_t b = blockIdx.x;
-    const int* pv = piv + b * n;
+    const int* pm_in = piv + b * n;
     int* pm = perm + b * n;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sp[i] = i;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        sp[i] = i;
+        if (staged) sp[n + i] = pm_in[i];
+    }
+    const int* sv = staged ? sp + n : pm_in;
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = n - 1; k >= 0; --k) {
-            const int q = pv[k];
+            const int q = sv[k];
             const int x = sp[k]; sp[k] = sp[q]; sp[q] = x;
         }
     }
@@ -840,7 +847,9 @@ void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int
                 bool symmetrize, const GjWork& W, hipStream_t s) {
     if (n <= 0 || batch <= 0) return;
     gj_rec(mat, n, ld, stride, batch, 0, n, W, s);
-    hipLaunchKernelGGL(k_gj_perm, dim3(batch), dim3(256), sizeof(int) * n, s, W.piv, W.perm, n);
+    const int staged = (n <= 8192) ? 1 : 0;
+    hipLaunchKernelGGL(k_gj_perm, dim3(batch), dim3(256), (staged ? 2 : 1) * sizeof(int) * n, s, W.piv, W.perm, n,
+                       staged);
     dim3 fg((n + PT - 1) / PT, (n + PT - 1) / PT, batch);
     hipLaunchKernelGGL(k_gj_final, fg, dim3(256), 0, s, mat, out, ld, stride, n, W.perm, symmetrize ? 1 : 0);
 }
