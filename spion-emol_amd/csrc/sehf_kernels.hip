@@ -552,34 +552,60 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t
     __shared__ double s_rows[LEAF_BLK][GJW + 1];   // the block's panel rows (+1: bank spread)
     __shared__ int s_nom[4][GJW];
     __shared__ int s_last;
+    // The block's 256 x 16 panel tile (and, for the pre-update, its tile in
+    // the left sibling's columns) moves between HBM and registers with 16
+    // lanes per row segment, and is turned into one row per thread through
+    // LDS: a per-thread row access would touch 64 lines per instruction.
+    const int r0 = blockIdx.x * LEAF_BLK;
+    const int tj = t % GJW, tr = t / GJW;
+    constexpr int TRS = LEAF_BLK / GJW;   // tile rows per pass
+    double pv[GJW], px[GJW];
+#pragma unroll
+    for (int it = 0; it < GJW; ++it) {
+        const int rr = r0 + tr + it * TRS;
+        pv[it] = (rr < n && tj < w) ? M[(int64_t)rr * ld + c + tj] : 0.0;
+        px[it] = (rr < n && tj < pre_w) ? M[(int64_t)rr * ld + pre_c + tj] : 0.0;
+    }
+    if (pre_w > 0) s_T[t / GJW][t % GJW] = stg_T(stg + b * STG_D)[t];
+#pragma unroll
+    for (int it = 0; it < GJW; ++it) s_rows[tr + it * TRS][tj] = pv[it];
+    __syncthreads();
     double v[GJW];
 #pragma unroll
-    for (int j = 0; j < GJW; ++j) v[j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
+    for (int j = 0; j < GJW; ++j) v[j] = s_rows[t][j];
     if (pre_w > 0) {
-        const double* Tp = stg_T(stg + b * STG_D);
-        s_T[t / GJW][t % GJW] = Tp[t];
+        double xv[GJW];
         __syncthreads();
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) s_rows[tr + it * TRS][tj] = px[it];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) xv[j] = s_rows[t][j];
         if (r < n) {
-            const double* xr = M + (int64_t)r * ld + pre_c;
             const bool inK = (r >= pre_c && r < pre_c + pre_w);
             double acc[GJW];
 #pragma unroll
             for (int j = 0; j < GJW; ++j) acc[j] = inK ? 0.0 : v[j];
-#pragma unroll 4
-            for (int kk = 0; kk < pre_w; ++kk) {
-                const double x = xr[kk];
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) acc[j] = fma(x, s_T[kk][j], acc[j]);
+            for (int kk = 0; kk < GJW; ++kk) {
+                if (kk < pre_w) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) acc[j] = fma(xv[kk], s_T[kk][j], acc[j]);
+                }
             }
 #pragma unroll
-            for (int j = 0; j < GJW; ++j) {
-                v[j] = (j < w) ? acc[j] : 0.0;
-                if (j < w) M[(int64_t)r * ld + c + j] = v[j];
-            }
+            for (int j = 0; j < GJW; ++j) v[j] = (j < w) ? acc[j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) s_rows[t][j] = v[j];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) {
+            const int rr = r0 + tr + it * TRS;
+            if (rr < n && tj < w) M[(int64_t)rr * ld + c + tj] = s_rows[tr + it * TRS][tj];
         }
     }
-#pragma unroll
-    for (int j = 0; j < GJW; ++j) s_rows[t][j] = v[j];
     // wave-local partial pivoting over the wave's rows (r >= c eligible)
     {
         double cv[1][GJW];
@@ -627,28 +653,61 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
     __shared__ double s_inv[GJW][GJW];
     __shared__ double s_T2[GJW][GJW];
     __shared__ int s_dst[2 * GJW], s_src[2 * GJW];
-    // own row's panel and side-range values, loaded with the staging data
-    // (replaced from the staging area below when the row receives a move)
-    const int r = blockIdx.x * LEAF_BLK + t;
-    double a0[GJW], x0[GJW];
+    __shared__ double s_tile[LEAF_BLK][GJW + 1];
+    // The block's panel tile and side-range tile move between HBM and
+    // registers with 16 lanes per row segment and are turned into one row
+    // per thread (and back) through LDS, as in k_leaf_nominate.
+    const int r0 = blockIdx.x * LEAF_BLK, r = r0 + t;
+    const int tj = t % GJW, tr = t / GJW;
+    constexpr int TRS = LEAF_BLK / GJW;
+    double pa[GJW], px[GJW];
 #pragma unroll
-    for (int j = 0; j < GJW; ++j) {
-        a0[j] = (r < n && j < w) ? M[(int64_t)r * ld + c + j] : 0.0;
-        x0[j] = (post && r < n && j < xw) ? M[(int64_t)r * ld + xa0 + j] : 0.0;
+    for (int it = 0; it < GJW; ++it) {
+        const int rr = r0 + tr + it * TRS;
+        pa[it] = (rr < n && tj < w) ? M[(int64_t)rr * ld + c + tj] : 0.0;
+        px[it] = (post && rr < n && tj < xw) ? M[(int64_t)rr * ld + xa0 + tj] : 0.0;
     }
     s_inv[t / GJW][t % GJW] = stg_ainv(S)[t];
     if (post) s_T2[t / GJW][t % GJW] = stg_T(S)[t];
     if (t < 2 * GJW) { s_dst[t] = SI[t]; s_src[t] = SI[2 * GJW + t]; }
     const int cnt = SI[4 * GJW];
+#pragma unroll
+    for (int it = 0; it < GJW; ++it) s_tile[tr + it * TRS][tj] = pa[it];
     __syncthreads();
-    // ---- rows
+    // Columns outside the panel (and outside Jq when post) only receive the
+    // row moves; their loads are issued first so they are in flight during
+    // the row work (the two touch disjoint columns).
+    const int jc = blockIdx.x * LEAF_BLK + t;
+    const bool colmv = cnt > 0 && jc < n && !(jc >= c && jc < c + w) && !(post && jc >= xa0 && jc < xa0 + xw);
+    double xm[2 * GJW];
+    if (colmv) {
+#pragma unroll
+        for (int e = 0; e < 2 * GJW; ++e)
+            if (e < cnt) xm[e] = M[(int64_t)s_src[e] * ld + jc];
+    }
+    // own row's panel and side-range values (replaced from the staging area
+    // below when the row receives a move)
+    double a0[GJW], x0[GJW];
+#pragma unroll
+    for (int j = 0; j < GJW; ++j) a0[j] = s_tile[t][j];
+    if (post) {
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) s_tile[tr + it * TRS][tj] = px[it];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) x0[j] = s_tile[t][j];
+    }
+    // ---- rows: o = new panel values, x = new side-range values
+    double o[GJW], x[GJW];
+#pragma unroll
+    for (int j = 0; j < GJW; ++j) { o[j] = 0.0; x[j] = 0.0; }
     if (r < n) {
         int e_src = -1;
 #pragma unroll
         for (int e = 0; e < 2 * GJW; ++e)
             if (e < cnt && s_dst[e] == r) e_src = e;
         const bool inK = (r >= c && r < c + w);
-        double o[GJW];
         if (inK) {
 #pragma unroll
             for (int j = 0; j < GJW; ++j) o[j] = s_inv[r - c][j];
@@ -662,27 +721,20 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
                 for (int j = 0; j < GJW; ++j) a[j] = a0[j];
             }
 #pragma unroll
-            for (int j = 0; j < GJW; ++j) o[j] = 0.0;
-#pragma unroll
             for (int kk = 0; kk < GJW; ++kk) {
 #pragma unroll
                 for (int j = 0; j < GJW; ++j) o[j] = fma(-a[kk], s_inv[kk][j], o[j]);
             }
         }
-#pragma unroll
-        for (int j = 0; j < GJW; ++j)
-            if (j < w) M[(int64_t)r * ld + c + j] = o[j];
         if (post) {
-            double x[GJW];
-            if (inK) {
+            if (!inK) {
+                if (e_src >= 0) {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) x[j] = 0.0;
-            } else if (e_src >= 0) {
+                    for (int j = 0; j < GJW; ++j) x[j] = stg_mvX(S)[e_src * GJW + j];
+                } else {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) x[j] = stg_mvX(S)[e_src * GJW + j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < GJW; ++j) x[j] = x0[j];
+                    for (int j = 0; j < GJW; ++j) x[j] = x0[j];
+                }
             }
 #pragma unroll
             for (int kk = 0; kk < GJW; ++kk) {
@@ -691,21 +743,33 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
                     for (int j = 0; j < GJW; ++j) x[j] = fma(o[kk], s_T2[kk][j], x[j]);
                 }
             }
-#pragma unroll
-            for (int j = 0; j < GJW; ++j)
-                if (j < xw) M[(int64_t)r * ld + xa0 + j] = x[j];
         }
     }
-    // ---- columns outside the panel (and outside Jq when post): row moves
-    const int j = blockIdx.x * LEAF_BLK + t;
-    if (cnt > 0 && j < n && !(j >= c && j < c + w) && !(post && j >= xa0 && j < xa0 + xw)) {
-        double x[2 * GJW];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < GJW; ++j) s_tile[t][j] = o[j];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < GJW; ++it) {
+        const int rr = r0 + tr + it * TRS;
+        if (rr < n && tj < w) M[(int64_t)rr * ld + c + tj] = s_tile[tr + it * TRS][tj];
+    }
+    if (post) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) s_tile[t][j] = x[j];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) {
+            const int rr = r0 + tr + it * TRS;
+            if (rr < n && tj < xw) M[(int64_t)rr * ld + xa0 + tj] = s_tile[tr + it * TRS][tj];
+        }
+    }
+    // ---- the row moves of the other columns
+    if (colmv) {
 #pragma unroll
         for (int e = 0; e < 2 * GJW; ++e)
-            if (e < cnt) x[e] = M[(int64_t)s_src[e] * ld + j];
-#pragma unroll
-        for (int e = 0; e < 2 * GJW; ++e)
-            if (e < cnt) M[(int64_t)s_dst[e] * ld + j] = x[e];
+            if (e < cnt) M[(int64_t)s_dst[e] * ld + jc] = xm[e];
     }
 }
 
