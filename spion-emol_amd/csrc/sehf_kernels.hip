@@ -537,8 +537,10 @@ __device__ __forceinline__ void leaf_merge_wave(const double* __restrict__ M, in
 // Nominate (all blocks) + merge (the block that finishes last, found with a
 // per-matrix completion counter; no block waits for another).  Each block
 // publishes its nominees and pre-updated panel rows with a device-scope
-// fence before counting in; the last block fences again before reading
-// them and resets the counter for the next leaf.
+// release fence before counting in; only the last block, which reads them,
+// takes the acquire fence (it invalidates the XCD's L2, which the other
+// blocks have no reason to pay for), and it resets the counter for the next
+// leaf.
 template <int NC>
 __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t ld, int64_t stride, int n,
                                                             int c, int w, int pre_c, int pre_w, int xa0,
@@ -628,12 +630,12 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_nominate(double* mat, int64_t
         if (lane < GJW) nom[(b * nblk + blockIdx.x) * GJW + lane] = (lane < w) ? ch : -1;
     }
     // count in; the last block of this matrix runs the merge
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (t == 0) s_last = (atomicAdd(cnt + b, 1) == nblk - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (wv == 0)
         leaf_merge_wave<NC>(M, ld, n, c, w, xa0, xw, nom + b * nblk * GJW, nblk, stg + b * STG_D,
                             stgi + b * STG_I, piv + b * n, info + b, lane);
