@@ -654,8 +654,29 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
     const int* SI = stgi + b * STG_I;
     __shared__ double s_inv[GJW][GJW];
     __shared__ double s_T2[GJW][GJW];
-    __shared__ int s_dst[2 * GJW], s_src[2 * GJW];
+    __shared__ double s_mv[2][2 * GJW][GJW];   // moved rows' old panel / side-range values
     __shared__ double s_tile[LEAF_BLK][GJW + 1];
+    // Staging first: the loads below are waited for in issue order, so the
+    // small staging reads must not queue behind the tile and row-move loads.
+    constexpr int NMV = 4 * GJW * GJW / LEAF_BLK;
+    const double inv_v = stg_ainv(S)[t];
+    const double t2_v = post ? stg_T(S)[t] : 0.0;
+    double mvv[NMV];
+#pragma unroll
+    for (int q = 0; q < NMV; ++q) {
+        const int idx = t + q * LEAF_BLK, h = idx / (2 * GJW * GJW), e = idx % (2 * GJW * GJW);
+        mvv[q] = (h == 0 ? stg_mvK(S) : stg_mvX(S))[e];
+    }
+    // The move list is uniform.  Slots at or beyond cnt repeat move 0 (the
+    // same value to the same address), which keeps the row-move loads and
+    // stores below free of per-slot branches.
+    const int cnt = SI[4 * GJW];
+    int mdst[2 * GJW], msrc[2 * GJW];
+#pragma unroll
+    for (int e = 0; e < 2 * GJW; ++e) { mdst[e] = SI[e]; msrc[e] = SI[2 * GJW + e]; }
+#pragma unroll
+    for (int e = 1; e < 2 * GJW; ++e)
+        if (e >= cnt) { mdst[e] = mdst[0]; msrc[e] = msrc[0]; }
     // The block's panel tile and side-range tile move between HBM and
     // registers with 16 lanes per row segment and are turned into one row
     // per thread (and back) through LDS, as in k_leaf_nominate.
@@ -669,23 +690,30 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
         pa[it] = (rr < n && tj < w) ? M[(int64_t)rr * ld + c + tj] : 0.0;
         px[it] = (post && rr < n && tj < xw) ? M[(int64_t)rr * ld + xa0 + tj] : 0.0;
     }
-    s_inv[t / GJW][t % GJW] = stg_ainv(S)[t];
-    if (post) s_T2[t / GJW][t % GJW] = stg_T(S)[t];
-    if (t < 2 * GJW) { s_dst[t] = SI[t]; s_src[t] = SI[2 * GJW + t]; }
-    const int cnt = SI[4 * GJW];
-#pragma unroll
-    for (int it = 0; it < GJW; ++it) s_tile[tr + it * TRS][tj] = pa[it];
-    __syncthreads();
     // Columns outside the panel (and outside Jq when post) only receive the
-    // row moves; their loads are issued first so they are in flight during
-    // the row work (the two touch disjoint columns).
+    // row moves: their loads go out with the tile loads and their stores
+    // follow the first barrier (they touch columns disjoint from the tiles
+    // this kernel reads and writes).
     const int jc = blockIdx.x * LEAF_BLK + t;
     const bool colmv = cnt > 0 && jc < n && !(jc >= c && jc < c + w) && !(post && jc >= xa0 && jc < xa0 + xw);
     double xm[2 * GJW];
     if (colmv) {
 #pragma unroll
-        for (int e = 0; e < 2 * GJW; ++e)
-            if (e < cnt) xm[e] = M[(int64_t)s_src[e] * ld + jc];
+        for (int e = 0; e < 2 * GJW; ++e) xm[e] = M[(int64_t)msrc[e] * ld + jc];
+    }
+    s_inv[t / GJW][t % GJW] = inv_v;
+    if (post) s_T2[t / GJW][t % GJW] = t2_v;
+#pragma unroll
+    for (int q = 0; q < NMV; ++q) {
+        const int idx = t + q * LEAF_BLK, h = idx / (2 * GJW * GJW), e = idx % (2 * GJW * GJW);
+        s_mv[h][e / GJW][e % GJW] = mvv[q];
+    }
+#pragma unroll
+    for (int it = 0; it < GJW; ++it) s_tile[tr + it * TRS][tj] = pa[it];
+    __syncthreads();
+    if (colmv) {
+#pragma unroll
+        for (int e = 0; e < 2 * GJW; ++e) M[(int64_t)mdst[e] * ld + jc] = xm[e];
     }
     // own row's panel and side-range values (replaced from the staging area
     // below when the row receives a move)
@@ -708,7 +736,7 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
         int e_src = -1;
 #pragma unroll
         for (int e = 0; e < 2 * GJW; ++e)
-            if (e < cnt && s_dst[e] == r) e_src = e;
+            if (e < cnt && mdst[e] == r) e_src = e;
         const bool inK = (r >= c && r < c + w);
         if (inK) {
 #pragma unroll
@@ -717,7 +745,7 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
             double a[GJW];
             if (e_src >= 0) {
 #pragma unroll
-                for (int j = 0; j < GJW; ++j) a[j] = stg_mvK(S)[e_src * GJW + j];
+                for (int j = 0; j < GJW; ++j) a[j] = s_mv[0][e_src][j];
             } else {
 #pragma unroll
                 for (int j = 0; j < GJW; ++j) a[j] = a0[j];
@@ -732,7 +760,7 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
             if (!inK) {
                 if (e_src >= 0) {
 #pragma unroll
-                    for (int j = 0; j < GJW; ++j) x[j] = stg_mvX(S)[e_src * GJW + j];
+                    for (int j = 0; j < GJW; ++j) x[j] = s_mv[1][e_src][j];
                 } else {
 #pragma unroll
                     for (int j = 0; j < GJW; ++j) x[j] = x0[j];
@@ -766,12 +794,6 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
             const int rr = r0 + tr + it * TRS;
             if (rr < n && tj < xw) M[(int64_t)rr * ld + xa0 + tj] = s_tile[tr + it * TRS][tj];
         }
-    }
-    // ---- the row moves of the other columns
-    if (colmv) {
-#pragma unroll
-        for (int e = 0; e < 2 * GJW; ++e)
-            if (e < cnt) M[(int64_t)s_dst[e] * ld + jc] = xm[e];
     }
 }
 
