@@ -7,6 +7,7 @@ results.
 """
 import numpy as np
 import pytest
+import torch
 
 from sehf import dipole_io as dio
 from sehf import observables, scan, synth
@@ -43,5 +44,5 @@ def test_scan_to_files_round_trip(tmp_path):
     a = observables.from_scan([{"k": k, "D": D} for k, D in zip(kf, Df)], E_homo)
     b = observables.from_scan([{"k": r["k"], "D": r["D"]} for r in res], E_homo)
     for key in ("sigma_L", "sigma_V"):
-        assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), key
+        assert torch.equal(a[key], b[key]), key
     assert np.array_equal(Df, Dr)
