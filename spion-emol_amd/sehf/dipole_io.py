@@ -360,3 +360,79 @@ def read_energy(scan_dir: str, ik: int) -> dict:
         r.update({"D": np.array(D), "D_raw": np.array(DR), "d_raw": np.array(dr), "d_corr": np.array(dc),
                   "partial_sigma": np.array(ps), "types": F.types})
     return r
+
+
+# ---------------------------------------------------------------------------
+# Post-processing front end: the per-channel .dat files of
+# postprocessing/gather_dipoles.py, and the files as Scan.run-shaped results.
+# ---------------------------------------------------------------------------
+def idx_to_lm(mu: int) -> tuple:
+    """Channel index of the real-Y basis, mu = l^2 + l + m -> (l, m)."""
+    l = int(np.floor(np.sqrt(mu)))
+    while (l + 1) * (l + 1) <= mu:
+        l += 1
+    while l * l > mu:
+        l -= 1
+    return l, mu - l * l - l
+
+
+def load_results(scan_dir: str, use_raw: bool = False) -> tuple:
+    """(manifest, results): one dict per written energy in ascending ik with
+    k, E, omega and D (6, n_ch) complex (D_raw instead when use_raw), the
+    input sehf.observables.from_scan takes."""
+    meta = read_manifest(scan_dir)
+    out = []
+    for ik in available_ik(scan_dir):
+        r = read_energy(scan_dir, ik)
+        out.append({"ik": ik, "k": r["k"], "E": r["E"], "omega": r["omega"], "D": r["D_raw"] if use_raw else r["D"]})
+    if not out:
+        raise RuntimeError(f"no ikNNNN.h5 files under {scan_dir}")
+    n = out[0]["D"].shape[1]
+    for r in out:
+        if r["D"].shape[1] != n:
+            raise RuntimeError(f"channel count differs between ik files (ik={r['ik']}: {r['D'].shape[1]} vs {n})")
+    return meta, out
+
+
+def gather(scan_dir: str, output_dir: str, use_raw: bool = False) -> int:
+    """Write dipole_{len,vel}_homo_{x,y,z}_{mu}.dat, one file per gauge,
+    polarisation and channel, with columns k_elec, E_kin, omega, Re D, Im D,
+    |D|^2, arg D over the energies (postprocessing/gather_dipoles.py:148-209),
+    so its consumer postprocessing/cross_section_delay.py reads them
+    unchanged.  Returns the number of files written."""
+    meta, res = load_results(scan_dir, use_raw)
+    os.makedirs(output_dir, exist_ok=True)
+    k = np.array([r["k"] for r in res])
+    E = np.array([r["E"] for r in res])
+    om = np.array([r["omega"] for r in res])
+    D = np.stack([r["D"] for r in res])          # [n_e, 6, n_ch]
+    head = (f"scan_dir: {scan_dir}\n"
+            f"dk = {meta['dk']:.12g} au   E_HOMO = {meta['E_HOMO']:.6f} au   "
+            f"Ip = {-meta['E_HOMO']:.6f} au   dipole = {'RAW (pre-ortho)' if use_raw else 'ORTHO'}\n"
+            "Columns: k_elec[au]  E_kin[au]  omega[au]  Re(D)  Im(D)  |D|^2  arg(D)")
+    n = 0
+    for s, (prefix, pol) in enumerate([(g, p) for g in ("dipole_len_homo", "dipole_vel_homo") for p in "xyz"]):
+        for mu in range(D.shape[2]):
+            l, m = idx_to_lm(mu)
+            d = D[:, s, mu]
+            cols = np.column_stack([k, E, om, d.real, d.imag, np.abs(d) ** 2, np.angle(d)])
+            np.savetxt(os.path.join(output_dir, f"{prefix}_{pol}_{mu}.dat"), cols,
+                       header=head + f"\nmu = {mu}  (l={l}, m={m:+d})", fmt="%.14e")
+            n += 1
+    return n
+
+
+def _main(argv=None) -> None:
+    import argparse
+    ap = argparse.ArgumentParser(description="Gather per-channel dipole files from a scan directory.")
+    ap.add_argument("scan_dir")
+    ap.add_argument("--output-dir", default=None, help="default: ./gathered_<scan_dir name>")
+    ap.add_argument("--raw", action="store_true", help="D_raw (before orthogonalisation) instead of D_ortho")
+    a = ap.parse_args(argv)
+    out = a.output_dir or os.path.join(os.getcwd(), "gathered_" + os.path.basename(os.path.normpath(a.scan_dir)))
+    n = gather(a.scan_dir, out, a.raw)
+    print(f"wrote {n} .dat files to {out}")
+
+
+if __name__ == "__main__":
+    _main()

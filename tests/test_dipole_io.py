@@ -182,3 +182,76 @@ def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(dio.IO_LIB_PATH)
     for n in names:
         assert hasattr(L, n), n
+
+
+def test_idx_to_lm():
+    seen = []
+    for l in range(12):
+        for m in range(-l, l + 1):
+            seen.append((l, m))
+    assert [dio.idx_to_lm(mu) for mu in range(len(seen))] == seen
+
+
+def test_gather_dat_files(tmp_path):
+    """The per-channel files of postprocessing/gather_dipoles.py: one per
+    (gauge, polarisation, channel); the columns reproduce k, E, omega and D
+    to the 15 digits of %.14e, |D|^2 and arg D."""
+    meta = ref_meta(l_max_continuum=2)
+    scan = str(tmp_path / "g")
+    iks = (5, 2, 9)
+    res = [payload(9, 5, k=ik * meta.dk, seed=ik) for ik in iks]
+    dio.write_scan(scan, meta, res)
+    out = str(tmp_path / "gathered")
+    assert dio.gather(scan, out) == 6 * 9
+    order = np.argsort(iks)
+    D = np.stack([res[i]["D"] for i in order])
+    for s, name in enumerate(["len_homo_x", "len_homo_y", "len_homo_z", "vel_homo_x", "vel_homo_y", "vel_homo_z"]):
+        for mu in range(9):
+            f = os.path.join(out, f"dipole_{name}_{mu}.dat")
+            head = [ln for ln in open(f) if ln.startswith("#")]
+            l, m = dio.idx_to_lm(mu)
+            assert head[-1].strip() == f"# mu = {mu}  (l={l}, m={m:+d})"
+            assert "dipole = ORTHO" in head[1]
+            c = np.loadtxt(f)
+            k = np.array(sorted(iks)) * meta.dk
+            assert np.allclose(c[:, 0], k, rtol=1e-14, atol=0)
+            assert np.allclose(c[:, 1], 0.5 * k * k, rtol=1e-14, atol=0)
+            assert np.allclose(c[:, 2], 0.5 * k * k - meta.E_HOMO, rtol=1e-14, atol=0)
+            d = D[:, s, mu]
+            assert np.allclose(c[:, 3] + 1j * c[:, 4], d, rtol=1e-14, atol=0)
+            assert np.allclose(c[:, 5], d.real ** 2 + d.imag ** 2, rtol=1e-13, atol=0)
+            assert np.allclose(c[:, 6], np.arctan2(d.imag, d.real), rtol=1e-13, atol=0)
+    _, raw = dio.load_results(scan, use_raw=True)
+    assert np.array_equal(raw[0]["D"], res[1]["D_raw"])      # ik 2 first
+
+
+def test_gathered_files_give_the_same_cross_sections(tmp_path):
+    """The .dat files, loaded the way postprocessing/cross_section_delay.py
+    loads them (np.loadtxt per channel, D = Re + i Im, channels of x, y, z
+    side by side; :167-192), give the cross sections of the numpy
+    restatement equal to sehf.observables on the in-memory results."""
+    import sys
+    import torch
+    from conftest import ROOT
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import observables_ref as OR
+    from sehf import observables
+    meta = ref_meta(l_max_continuum=2)
+    scan = str(tmp_path / "x")
+    res = [payload(9, 5, k=ik * meta.dk, seed=10 + ik) for ik in range(3, 9)]
+    dio.write_scan(scan, meta, res)
+    out = str(tmp_path / "gx")
+    dio.gather(scan, out)
+
+    def load(prefix):
+        cols = [np.loadtxt(os.path.join(out, f"{prefix}_{pol}_{mu}.dat")) for pol in "xyz" for mu in range(9)]
+        return cols[0][:, 0], cols[0][:, 2], np.stack([c[:, 3] + 1j * c[:, 4] for c in cols], axis=1)
+
+    kL, omL, DL = load("dipole_len_homo")
+    kV, omV, DV = load("dipole_vel_homo")
+    sL = OR.sigma_from_dipole(kL, DL, omL, "length")
+    sV = OR.sigma_from_dipole(kV, DV, omV, "velocity")
+    got = observables.from_scan(res, meta.E_HOMO, device="cpu")
+    assert np.allclose(got["sigma_L"].numpy(), sL, rtol=1e-13, atol=0)
+    assert np.allclose(got["sigma_V"].numpy(), sV, rtol=1e-13, atol=0)
+    assert isinstance(got["sigma_L"], torch.Tensor)
