@@ -273,8 +273,29 @@ void gemm(const Gemm& g, hipStream_t s) {
     const int pol = g_tiles.load(std::memory_order_relaxed);
     const bool big = g.M > 64 && (pol == 2 || (pol == 0 && g.K >= 128));
     if (big) {
-        dim3 grid((g.M + 127) / 128, (g.N + 127) / 128, g.batch);
-        hipLaunchKernelGGL((k_dgemm<128, 128, 2, 4>), grid, dim3(512), 0, s, p);
+        // Rows past the last full 128-row tile: when at most 64 remain, they
+        // go to a second launch with a thin tile instead of a mostly empty
+        // 128-row tile (392 rows: 400 rows of MFMA work instead of 512).
+        // Every tile shape sums K in the same order, so C is bitwise the same.
+        const int M1 = (g.M / 128) * 128, R = g.M - M1;
+        const bool split = M1 > 0 && R > 0 && R <= 64;
+        GemmK p1 = p;
+        if (split) p1.M = M1;
+        dim3 grid((p1.M + 127) / 128, (g.N + 127) / 128, g.batch);
+        hipLaunchKernelGGL((k_dgemm<128, 128, 2, 4>), grid, dim3(512), 0, s, p1);
+        if (split) {
+            GemmK p2 = p;
+            p2.M = R;
+            p2.A = g.tA ? g.A + static_cast<int64_t>(M1) * g.lda : g.A + M1;
+            p2.C = g.C + M1;
+            if (R <= 16) {
+                hipLaunchKernelGGL((k_dgemm<16, 128, 1, 4>), dim3(1, (g.N + 127) / 128, g.batch), dim3(256), 0, s, p2);
+            } else if (R <= 32) {
+                hipLaunchKernelGGL((k_dgemm<32, 128, 1, 4>), dim3(1, (g.N + 127) / 128, g.batch), dim3(256), 0, s, p2);
+            } else {
+                hipLaunchKernelGGL((k_dgemm<64, 64, 2, 2>), dim3(1, (g.N + 63) / 64, g.batch), dim3(256), 0, s, p2);
+            }
+        }
     } else if (pol != 1 && g.M <= 16) {
         dim3 grid((g.M + 15) / 16, (g.N + 127) / 128, g.batch);
         hipLaunchKernelGGL((k_dgemm<16, 128, 1, 4>), grid, dim3(256), 0, s, p);
