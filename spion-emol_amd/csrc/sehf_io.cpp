@@ -102,9 +102,13 @@ hid_t group(hid_t loc, const char* path) { return H5Gcreate2(loc, path, H5P_DEFA
 template <class F>
 void write_atomic(const std::string& final_path, F body) {
     const std::string tmp = final_path + ".tmp.pid" + std::to_string(static_cast<long long>(::getpid()));
-    {
+    try {
         H f(H5Fcreate(tmp.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose, "H5Fcreate " + tmp);
         body(f.id);
+    } catch (...) {   // no partial file is left behind
+        std::error_code rm;
+        fs::remove(tmp, rm);
+        throw;
     }
     std::error_code ec;
     fs::rename(tmp, final_path, ec);

@@ -255,3 +255,21 @@ def test_gathered_files_give_the_same_cross_sections(tmp_path):
     assert np.allclose(got["sigma_L"].numpy(), sL, rtol=1e-13, atol=0)
     assert np.allclose(got["sigma_V"].numpy(), sV, rtol=1e-13, atol=0)
     assert isinstance(got["sigma_L"], torch.Tensor)
+
+
+def test_failed_write_leaves_no_file(tmp_path):
+    """A write that fails inside HDF5 leaves neither the target nor its
+    temporary file (here the temporary name is taken by an empty directory,
+    so H5Fcreate fails), and a payload of inconsistent shapes is refused
+    before any file is created."""
+    meta = ref_meta()
+    scan = str(tmp_path / "f")
+    dio.open_scan(scan, meta)
+    os.makedirs(os.path.join(scan, "ik0004.h5.tmp.pid%d" % os.getpid()))
+    with pytest.raises(RuntimeError, match="H5Fcreate"):
+        dio.write_energy(scan, meta, 4, payload(4, 2, k=4 * meta.dk))
+    assert not dio.has_energy(scan, 4)
+    assert sorted(os.listdir(scan)) == ["manifest.h5"]
+    with pytest.raises(ValueError):
+        dio.write_energy(scan, meta, 4, {**payload(4, 2, k=4 * meta.dk), "d_corr": np.zeros((6, 3))})
+    assert sorted(os.listdir(scan)) == ["manifest.h5"]
