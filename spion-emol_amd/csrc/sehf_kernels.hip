@@ -5,11 +5,13 @@
 //                    4 waves for shallow ones, 16x128 / 32x128 for thin ones;
 //                    K staged through LDS in slabs of 16; XCD-aware tile order.
 //  * Gauss-Jordan  : in-place inversion with partial pivoting, recursive over
-//                    column blocks.  Leaves (<=16 columns) are factored by one
-//                    1024-thread workgroup per matrix with the panel held in
-//                    registers; each pivot swaps the two full rows at once so
-//                    no deferred row interchanges exist.  Interior nodes are
-//                    one MFMA GEMM over all rows (rank-w block update).
+//                    column blocks.  Leaves (<=16 columns) pick their pivots by
+//                    tournament over blocks of 256 rows (k_leaf_nominate, whose
+//                    last block merges) and apply them in a second launch
+//                    (k_leaf_apply); the pivot rows' interchanges are applied
+//                    in full at once, so no deferred row interchanges exist.
+//                    Interior nodes are one MFMA GEMM over all rows (rank-w
+//                    block update).
 //
 // Gauss-Jordan works on the row-major view of the column-major buffer
 // (element (i,j) at base[i*ld + j], i.e. on M^T).  Row pivoting of M^T is
