@@ -407,27 +407,24 @@ __device__ __forceinline__ int wave_select(double (&cv)[NC][GJW], const int (&cr
             if (bal != 0ull) {
                 const int L = __ffsll(static_cast<long long>(bal)) - 1;
                 const int ep = __builtin_amdgcn_readlane(eb, L);
-                int rowp = __builtin_amdgcn_readlane(crow[0], L);
+                // the winning slot ep is uniform: every lane selects that
+                // slot (a chain over the static slot indices, so the register
+                // array is never indexed dynamically), then lane L's value is
+                // read out once
+                int cr = crow[0];
 #pragma unroll
-                for (int e = 1; e < NC; ++e) {
-                    const int y = __builtin_amdgcn_readlane(crow[e], L);
-                    rowp = (ep == e) ? y : rowp;
-                }
+                for (int e = 1; e < NC; ++e) cr = (ep == e) ? crow[e] : cr;
+                const int rowp = __builtin_amdgcn_readlane(cr, L);
                 if (lane == L) csel |= 1u << ep;
                 if (lane == jj) chosen = rowp;
-                // pivot row: readlane of every slot, then a uniform select
-                // (selecting first would index the register array dynamically)
                 double prow[GJW];
 #pragma unroll
                 for (int j = 0; j < GJW; ++j) {
                     if (j >= jj) {
-                        double x = lane_read(cv[0][j], L);
+                        double x = cv[0][j];
 #pragma unroll
-                        for (int e = 1; e < NC; ++e) {
-                            const double y = lane_read(cv[e][j], L);
-                            x = (ep == e) ? y : x;
-                        }
-                        prow[j] = x;
+                        for (int e = 1; e < NC; ++e) x = (ep == e) ? cv[e][j] : x;
+                        prow[j] = lane_read(x, L);
                     } else {
                         prow[j] = 0.0;
                     }
