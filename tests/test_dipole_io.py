@@ -273,3 +273,20 @@ def test_failed_write_leaves_no_file(tmp_path):
     with pytest.raises(ValueError):
         dio.write_energy(scan, meta, 4, {**payload(4, 2, k=4 * meta.dk), "d_corr": np.zeros((6, 3))})
     assert sorted(os.listdir(scan)) == ["manifest.h5"]
+
+
+def test_gather_command_line(tmp_path, capsys):
+    """python -m sehf.dipole_io <scan_dir> [--output-dir] [--raw], the
+    replacement of postprocessing/gather_dipoles.py's command line."""
+    meta = ref_meta(l_max_continuum=1)
+    scan = str(tmp_path / "c")
+    res = [payload(4, 5, k=ik * meta.dk, seed=ik) for ik in (2, 3)]
+    dio.write_scan(scan, meta, res)
+    out = str(tmp_path / "o")
+    dio._main([scan, "--output-dir", out, "--raw"])
+    assert "wrote 24 .dat files" in capsys.readouterr().out
+    f = os.path.join(out, "dipole_vel_homo_z_3.dat")
+    head = [ln for ln in open(f) if ln.startswith("#")]
+    assert "dipole = RAW (pre-ortho)" in head[1]
+    c = np.loadtxt(f)
+    assert np.allclose(c[:, 3] + 1j * c[:, 4], [r["D_raw"][5, 3] for r in res], rtol=1e-14, atol=0)
