@@ -1,23 +1,39 @@
 #!/usr/bin/env python
 """Benchmark of the batched static-exchange photoionization scan on one MI355X.
 
-A step is one pass of the whole energy-dependent path -- Schur inverses,
-forward ratio recursion, K matrix, back-propagation, asymptotic fit and the
-six dipole slices -- over one batch of energies of the l_cont = 30 H2O-like
-scan (961 channels, N_total = 1566, 6001 radial points, k in [0.05, 2.0],
-closed channels allowed).  The metric is energy points per second with the
-energy-independent inputs already resident in HBM.
+Workload: the l_cont = 30 H2O-like scan (961 channels, N_total = 1566, 6001
+radial points at dr = 0.01, k in [0.05, 2.0], closed channels allowed), with
+the energy-independent inputs resident in HBM.
 
-Prints one JSON line (rank 0).  `roofline` is for the dominant kernel, the
-FP64 MFMA GEMM (every Gauss-Jordan block update and every back-propagation
-product): algorithmic flop of the sampled launches over their summed
-durations, timed with HIP events on the launch stream inside the timed steps.
-`cpu_baseline` times the CPU oracle (the restatement of the reference's FP64
-CPU path) on a bounded sample of radial steps and extrapolates to one energy.
+A scan takes each batch of B energies through three sweeps of the radial
+grid: the forward ratio recursion (Schur inverses one step ahead on a side
+stream), and, segment by segment from the top, the recompute of the
+segment's ratio matrices from its checkpoint and the back-propagation with
+the fused dipole and fit work; plus per-batch work (host tables, K matrix,
+fit pre-pass over the top segments, fit solve, dipole normalisation).
+
+A timed step is one segment cycle at full size (sehf_scan_run_segment): the
+B energies run segment s of all three sweeps -- L = ceil(sqrt(Nr-1)) = 78
+radial points of forward steps, of recompute and of back-propagation.  The
+timed steps visit segments spread evenly over the grid (inside and beyond
+the exchange cut-off), so their mean cost per radial point is the grid's.
+`value` (energy points per second) is B over the modelled time of one whole
+batch: the measured per-point cost of the cycles times the grid's Nr-1
+points, plus the fit pre-pass (recompute and back-propagation of the top
+segments, from the same measured per-point costs), plus the per-batch fixed
+stages timed on a full short-grid scan of the same channel sizes.
+
+Prints one JSON line.  `roofline` is for the dominant kernel, the FP64 MFMA
+GEMM k_dgemm (every Gauss-Jordan block update and every back-propagation
+product): the algorithmic flop of the sampled launches over their summed
+durations, timed with HIP events on each launch's own stream inside the
+timed steps.  `cpu_baseline` times the CPU oracle (the restatement of the
+reference's FP64 CPU path) on a bounded sample of radial steps.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -27,14 +43,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "spion-emol_amd"))
 
 PEAK_FP64_MFMA_TFLOPS = 78.6      # MI355X dense FP64 matrix peak (datasheet)
+METRIC = "energy points/sec @l_cont=30 (batched vs per-ik, 1 GPU); FP64-MFMA %; adjoint/fwd cost"
 
 # HBM bytes of one k_dgemm launch of the top-level Gauss-Jordan update of the
 # order-1566 inverse (783 x 1566 x 783, the largest single share of GEMM time
-# at l_cont = 30), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
-# spion-emol_amd/_build/gemm_probe and calibrated by a 1 GiB device copy
-# (profiles/k_dgemm_hbm_traffic_gemm_probe_b32.md).  batch -> (measured bytes,
-# algorithmic bytes).  Counters cannot be collected inside the scan itself.
+# at l_cont = 30): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
+# scaled by the same counters on a 1 GiB device copy (FETCH x 2, WRITE x 1).
+# batch -> (measured bytes, algorithmic bytes).  An external calibration run,
+# not a measurement of this process.
 TOP_UPDATE_TRAFFIC = {32: (1.7831e+09, 1.0987e+09), 48: (2.7581e+09, 1.6480e+09)}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
@@ -63,234 +84,289 @@ def tangent_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
     return float(fwd + bp + ex * (6 + n_occ) * 2 * Np ** 2)
 
 
+def spread_segments(n_seg, count, offset=0.5):
+    """`count` segment indices spread evenly over [0, n_seg)."""
+    return [min(n_seg - 1, int((i + offset) * n_seg / count)) for i in range(count)]
+
+
+def seg_points(lay, N, s):
+    L = lay["L"]
+    return min((s + 1) * L, N) - s * L
+
+
+def batch_seconds(lay, N, pts, t_fwd, t_rec, t_back, t_fixed):
+    """Modelled seconds of one whole batch from segment-cycle timings:
+    per-point costs of the three sweeps over the N = Nr-1 points, the fit
+    pre-pass (recompute of the top segments, back-propagation from the top
+    down to the fit window's first point) and the fixed per-batch stages."""
+    L, n_seg, nfs, s_pre = lay["L"], lay["n_seg"], lay["n_fit_lo"], lay["prepass_seg_lo"]
+    P = float(np.sum(pts))
+    f, r, b = np.sum(t_fwd) / P, np.sum(t_rec) / P, np.sum(t_back) / P
+    pre_rec = N - s_pre * L          # points recomputed by the pre-pass
+    pre_back = N - nfs               # points back-propagated by the pre-pass
+    sweeps = N * (f + r + b)
+    prepass = pre_rec * r + pre_back * b
+    return sweeps + prepass + t_fixed, {"per_point_s": {"forward": f, "recompute": r, "back": b},
+                                        "sweeps_s": sweeps, "prepass_s": prepass, "fixed_s": t_fixed}
+
+
+def fixed_stage_seconds(p, ks, B):
+    """Per-batch fixed stages (K matrix, fit solve, dipole normalisation,
+    download) from the stage timers of one whole scan of the same channel
+    sizes on a short grid (r_max = 3.2); they do not depend on the grid
+    length.  (The host tables are part of every timed segment cycle.)"""
+    from sehf import scan as sc, synth
+    q = synth.h2o_like(p.l_cont, p.dr, 3.2, p.n_occ, seed=1, allow_closed=True)
+    S = sc.Scan(q)
+    try:
+        t0 = time.time()
+        res = S.run(ks[:B], batch=B, groups=1)
+        wall = time.time() - t0
+        d = S.stats()
+        if not all(np.all(np.isfinite(r["D"])) for r in res):
+            raise SystemExit("non-finite dipole in the short-grid scan")
+    finally:
+        S.close()
+    fixed = d["kmatrix"] + d["fit_dipole"]
+    return fixed, {"short_grid_Nr": q.Nr, "scan_s": wall, "kmatrix_s": d["kmatrix"], "fit_dipole_s": d["fit_dipole"]}
+
+
+def cycle_rate(S, p, lay, ks, segs, warm, tangent=None):
+    """Segment cycles of the energies ks at the listed segments (after `warm`
+    untimed ones); returns the modelled batch seconds and the timings."""
+    N = p.Nr - 1
+    for s in warm:
+        S.run_segment(ks, s, tangent=tangent)
+    tf, tr, tb, pts, walls = [], [], [], [], []
+    for s in segs:
+        t0 = time.time()
+        t = S.run_segment(ks, s, tangent=tangent)
+        walls.append(time.time() - t0)
+        tf.append(t["forward"] + t["host_tables"]); tr.append(t["recompute"]); tb.append(t["back"])
+        pts.append(seg_points(lay, N, s))
+    return tf, tr, tb, pts, walls
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
 def cpu_baseline(p, ks, n_steps):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the
     host: one energy-independent setup, then `cores` energies concurrently
     (one thread each) run n_steps radial steps of Sinv + FRP + BP at full
-    size; extrapolated to Nr steps per energy."""
+    size; extrapolated to Nr steps per energy.  The reference keeps every
+    ratio matrix (disk or RAM), so its energy is Nr steps of each stage with
+    no recompute.  K, fit and dipole are O(N^3) once per energy (under 0.1%)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
-    cores = max(1, min(os.cpu_count() or 1, 16))
+    model, nproc = cpu_info()
+    cores = max(1, min(nproc, 16))
     sample = np.asarray(ks[:cores], dtype=np.float64)
     t, wall = O.time_steps_multi(p, sample, n_steps)
-    step_s = float(np.mean(t.sum(axis=1)))
+    per_stage = t.mean(axis=0)                     # seconds per radial step per energy
+    step_s = float(per_stage.sum())
     energy_s = step_s * p.Nr
-    return {"value": cores / energy_s, "unit": "energy points/s", "cores": cores, "kind": "port",
+    rate_hoisted = cores / energy_s
+    rate_per_ik_setup = cores / (energy_s + wall[0])
+    return {"value": rate_hoisted, "unit": "energy points/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "nproc": nproc,
             "sample": f"{n_steps} radial step(s) of Sinv+FRP+BP at full size (N_tot={p.n_tot}) for {cores} "
                       f"energies run concurrently, one thread each ({wall[1]:.1f} s wall after a {wall[0]:.1f} s "
                       f"energy-independent setup); {step_s:.2f} s per step per energy x {p.Nr} steps = "
-                      f"{energy_s:.0f} s per energy (extrapolated)"}
-
-
-def side_measurements(args, p, ks_all, B):
-    """Short-grid ratios (same channel sizes, r_max = args.side_rmax; per-step
-    costs do not depend on the grid length):
-    * batched vs the plain per-ik path on this configuration: B energies in
-      one batch against one energy alone (batch 1);
-    * the forward-mode derivative's cost on the l_cont = 20 polarisation
-      configuration: the same B energies with and without the tangent."""
-    import torch
-    from sehf import scan as sc, synth
-    out = {"r_max": args.side_rmax}
-    q = synth.h2o_like(p.l_cont, p.dr, args.side_rmax, p.n_occ, seed=1, allow_closed=True)
-    S = sc.Scan(q)
-    ks = ks_all[(np.arange(B) * 37) % len(ks_all)]
-
-    def timed(f):
-        torch.cuda.synchronize()
-        t0 = time.time()
-        f()
-        torch.cuda.synchronize()
-        return time.time() - t0
-
-    S.run(ks[:1], batch=1)                                    # warm the per-ik buffers
-    t_b = timed(lambda: S.run(ks, batch=B, groups=1))
-    t_1 = timed(lambda: S.run(ks[:1], batch=1, groups=1))
-    out["per_ik"] = {"workload": f"{args.config} channels, Nr={q.Nr}", "batched_energies_per_s": B / t_b,
-                     "per_ik_energies_per_s": 1.0 / t_1, "batched_over_per_ik": (B / t_b) * t_1}
-    print(f"[bench] side: batched {B / t_b:.3f} vs per-ik {1.0 / t_1:.3f} energies/s on Nr={q.Nr}",
-          file=sys.stderr, flush=True)
-    S.close()
-    pc, kc = synth.config("h2o_l20_pol")
-    q = synth.h2o_like(pc.l_cont, pc.dr, args.side_rmax, pc.n_occ, seed=1, pol=(pc.pol_alpha, pc.pol_rc),
-                       allow_closed=True)
-    S = sc.Scan(q)
-    ks = kc[(np.arange(B) * 37) % len(kc)]
-    S.run(ks, batch=B, tangent=(1.0, 0.0))                   # allocate the tangent buffers
-    t_f = timed(lambda: S.run(ks, batch=B))
-    t_t = timed(lambda: S.run(ks, batch=B, tangent=(1.0, 0.0)))
-    out["tangent"] = {"workload": f"h2o_l20_pol channels (N_tot={q.n_tot}), Nr={q.Nr}, batch {B}",
-                      "forward_energies_per_s": B / t_f, "with_tangent_energies_per_s": B / t_t,
-                      "tangent_over_forward_cost": t_t / t_f}
-    print(f"[bench] side: tangent/forward cost {t_t / t_f:.2f} on Nr={q.Nr}", file=sys.stderr, flush=True)
-    S.close()
-    return out
+                      f"{energy_s:.0f} s per energy (extrapolated)",
+            "stage_seconds_per_energy": {"sinv": per_stage[0] * p.Nr, "frp": per_stage[1] * p.Nr,
+                                         "bp": per_stage[2] * p.Nr},
+            "setup_seconds": wall[0],
+            "setup_hoisted_value": rate_hoisted,
+            "setup_per_ik_value": rate_per_ik_setup}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--batch", type=int, default=0,
-                    help="energies per batch; 0 = batch width, groups and GEMM tiles chosen by the library's "
-                         "autotuner (timed probes of radial steps, outside the timed region)")
-    ap.add_argument("--groups", type=int, default=1,
-                    help="concurrent energy groups (streams); a step is groups x batch energies")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=48,
+                    help="energies per batch (48 is the autotuner's choice on this configuration); 0 = run "
+                         "the library's autotuner first (timed probes, outside the timed region)")
+    ap.add_argument("--tiles", type=int, default=2, help="GEMM tile policy (sehf_set_gemm_tiles); -1 = library default")
     ap.add_argument("--config", default="h2o_l30")
-    ap.add_argument("--seg-len", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=4, help="radial steps in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--prof-every", type=int, default=97)
-    ap.add_argument("--per-ik", action="store_true",
-                    help="also time the plain per-ik path (batch 1, one group) on one energy of the full grid")
-
-    ap.add_argument("--tangent", action="store_true",
-                    help="time the batched scan with its forward-mode derivative along E (batched + adjoint)")
-    ap.add_argument("--side-rmax", type=float, default=5.0,
-                    help="radial extent of the short-grid side measurements (per-ik vs batched on this config; "
-                         "tangent vs plain on the l_cont=20 polarisation config); 0 skips them")
-    ap.add_argument("--rmax", type=float, default=0.0,
-                    help="profiling aid: shorten the radial grid (same channel sizes); not the metric workload")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="profiling aid: one device-to-device copy of this many MiB before the timed step, "
-                         "the known-size reference for HBM byte counters")
+    ap.add_argument("--no-side", action="store_true", help="skip the per-ik, derivative and fixed-stage runs")
+    ap.add_argument("--prof-every", type=int, default=97, help="HIP-event sampling of every n-th GEMM/leaf "
+                                                                "launch; 0 = off")
+    ap.add_argument("--full-scan", action="store_true",
+                    help="also run one whole batch through the full grid (minutes) and report its measured rate "
+                         "beside the modelled one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus above 1 is not supported "
-              "(the design shards energies across GPUs in contiguous k blocks; no collective is needed)", file=sys.stderr)
+              "(the design shards energies across GPUs in contiguous k blocks; no collective is needed)",
+              file=sys.stderr)
         sys.exit(2)
 
     import ctypes
-    import torch
     from sehf import scan as sc, synth
-    from sehf._lib import load
+    from sehf._lib import check, load
 
-    torch.cuda.set_device(0)
+    L = load()
     t0 = time.time()
     p, ks_all = synth.config(args.config)
-    if args.rmax > 0:
-        p = synth.h2o_like(p.l_cont, p.dr, args.rmax, p.n_occ, seed=1, allow_closed=True)
-    t_gen = time.time() - t0
-    print(f"[bench] {args.config}: N_psi={p.n_psi} N_f={p.n_f} N_tot={p.n_tot} Nr={p.Nr} "
-          f"(inputs generated in {t_gen:.1f} s)", file=sys.stderr, flush=True)
+    log(f"{args.config}: N_psi={p.n_psi} N_f={p.n_f} N_tot={p.n_tot} Nr={p.Nr} "
+        f"(inputs generated in {time.time() - t0:.1f} s)")
     t0 = time.time()
     S = sc.Scan(p)
     t_setup = time.time() - t0
     info = S.info
-    print(f"[bench] setup (V(r), exchange, dipole tables on device) {t_setup:.1f} s; info {info}",
-          file=sys.stderr, flush=True)
-
-    B = args.batch
-    G = max(1, args.groups)
-    tune = None
+    log(f"setup (V(r), exchange, dipole tables on device) {t_setup:.1f} s; info {info}")
+    if args.tiles >= 0:
+        check(L.sehf_set_gemm_tiles(args.tiles), "sehf_set_gemm_tiles")
+    B, tune = args.batch, None
+    nk = len(ks_all)
     if B <= 0:
         t0 = time.time()
-        tune = S.autotune(ks_all[(np.arange(min(128, len(ks_all))) * 37) % len(ks_all)], probe_steps=8,
-                          tangent=args.tangent)
-        B, G = tune["batch"], tune["groups"]
+        tune = S.autotune(ks_all[(np.arange(min(128, nk)) * 37) % nk], probe_steps=8)
+        B = tune["batch"]
         tune["seconds"] = time.time() - t0
-        print(f"[bench] autotune: batch {B}, groups {G} ({tune['seconds']:.1f} s)", file=sys.stderr, flush=True)
-    E_step = B * G
-    nk = len(ks_all)
+        log(f"autotune: batch {B}, groups {tune['groups']}, tiles {tune['tiles']} ({tune['seconds']:.1f} s)")
+    # one batch: a spread of the scan's k values (closed and open channels mixed)
+    ks = ks_all[(np.arange(B) * 37) % nk]
+    lay = S.layout()
+    N = p.Nr - 1
+    warm = spread_segments(lay["n_seg"], args.warmup, offset=0.25)
+    segs = spread_segments(lay["n_seg"], args.steps)
+    log(f"layout {lay}; warmup segments {warm}; timed segments {segs}")
 
-    def batch_k(i):
-        # a spread of the scan's k values (closed and open channels mixed)
-        idx = [((i * E_step + j) * 37) % nk for j in range(E_step)]
-        return ks_all[idx]
-
-    L = load()
-    if args.calib_copy_mib > 0:
-        n = args.calib_copy_mib * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        print(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written",
-              file=sys.stderr, flush=True)
-    tan = (1.0, 0.0) if args.tangent else None
-    for w in range(args.warmup):
-        S.run(batch_k(w), batch=B, seg_len=args.seg_len, groups=G, tangent=tan)
-        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    L.sehf_prof_begin(args.prof_every)
-    t0 = time.time()
-    outs = []
-    for i in range(args.steps):
-        outs.append(S.run(batch_k(args.warmup + i), batch=B, seg_len=args.seg_len, groups=G, tangent=tan))
-        print(f"[bench] timed step {i} done ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.time() - t0
+    for i, s in enumerate(warm):
+        S.run_segment(ks, s)
+        log(f"warmup step {i} (segment {s}) done")
+    if args.prof_every > 0:
+        check(L.sehf_prof_begin(args.prof_every), "sehf_prof_begin")
+    tf, tr, tb, pts, walls = [], [], [], [], []
+    t_start = time.time()
+    for i, s in enumerate(segs):
+        t = S.run_segment(ks, s)
+        tf.append(t["forward"] + t["host_tables"]); tr.append(t["recompute"]); tb.append(t["back"])
+        pts.append(seg_points(lay, N, s))
+        log(f"timed step {i} (segment {s}, {pts[-1]} points) {t['host_tables'] + t['forward'] + t['recompute'] + t['back']:.2f} s "
+            f"({time.time() - t_start:.1f} s)")
+    elapsed = time.time() - t_start
     prof = (ctypes.c_double * 16)()
-    L.sehf_prof_end(prof)
+    if args.prof_every > 0:
+        check(L.sehf_prof_end(prof), "sehf_prof_end")
     prof = list(prof)
-    energies = E_step * args.steps
-    value = energies / elapsed
-    mflop = model_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
-                                   info["n_transition"], info["n_start"], p.n_occ)
-    if args.tangent:
-        mflop += tangent_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
-                                          info["n_transition"], info["n_start"], p.n_occ)
+
     gemm_tflops = prof[1] / (prof[0] * 1e-3) / 1e12 if prof[0] > 0 else 0.0
     avg_launch_us = prof[0] / max(prof[2], 1) * 1e3
     buckets = {}
     for i, nm in enumerate(("gemm_all", "gj_leaf", "gemm_K_lt_64", "gemm_K_64_256", "gemm_K_ge_256")):
         ms, fl, n = prof[3 * i: 3 * i + 3]
         if n:
-            buckets[nm] = {"sampled": int(n), "avg_us": ms / n * 1e3, "tflops": fl / (ms * 1e-3) / 1e12,
-                           "est_total_s": ms * args.prof_every / 1e3}
-    stages = S.stats()
-    for o in outs:
-        for r in o:
-            if not np.all(np.isfinite(r["D"])) or (args.tangent and not np.all(np.isfinite(r["dD"]))):
-                raise SystemExit("non-finite dipole in bench output")
+            buckets[nm] = {"sampled": int(n), "avg_us": ms / n * 1e3, "tflops": fl / (ms * 1e-3) / 1e12}
 
-    traffic, traffic_note = None, None
-    if args.config == "h2o_l30" and args.rmax == 0 and B in TOP_UPDATE_TRAFFIC:
-        tb, ta = TOP_UPDATE_TRAFFIC[B]
-        traffic = tb
-        traffic_note = (f"HBM bytes of the 783x1566x783 Gauss-Jordan update launch at batch {B} "
-                        f"(rocprofv3 --pmc on gemm_probe, calibrated); algorithmic {ta:.3e} B, ratio {tb / ta:.2f}")
-    line = {
-        "metric": "energy points/sec @l_cont=30 (batched vs per-ik, 1 GPU); FP64-MFMA %; adjoint/fwd cost",
-        "value": value, "unit": "energy points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), "
-                               f"Nr={p.Nr}, dr={p.dr}, k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
-                               f"batch {B} energies per step, closed channels allowed"
-                               + (", with the forward-mode derivative along E (batched + adjoint)" if args.tangent
-                                  else ""),
-                   "batch_energies": B, "groups": G, "energies_per_step": E_step, "tangent": bool(args.tangent),
-                   "seg_len": args.seg_len or "ceil(sqrt(Nr))"},
-        "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS, "traffic": traffic,
-                     "traffic_launch": traffic_note,
-                     "kernel": "k_dgemm", "sampled_launches": int(prof[2]),
-                     "avg_launch_us": avg_launch_us, "event_pair_overhead_us": prof[15] * 1e3},
-        "model_tflop_per_energy": mflop / 1e12,
-        "path_tflops": mflop * energies / elapsed / 1e12,
-        "path_frac_of_fp64_peak": mflop * energies / elapsed / 1e12 / PEAK_FP64_MFMA_TFLOPS,
-        "stage_seconds": stages, "setup_seconds": t_setup,
-        "sampled_kernels": buckets, "prof_every": args.prof_every,
-    }
-    if args.rmax > 0:
-        line["config"]["workload"] += f" [profiling grid r_max={args.rmax}, not the metric workload]"
-    if args.per_ik:
+    side = {}
+    if not args.no_side:
+        # plain per-ik GPU path on the same configuration and handle: one
+        # energy per batch, the same segment cycles
+        k1 = ks[B // 2: B // 2 + 1]
+        few = spread_segments(lay["n_seg"], 4)
+        f1, r1, b1, p1, _ = cycle_rate(S, p, lay, k1, few, warm=[few[1]])
+        t1, _ = batch_seconds(lay, N, p1, f1, r1, b1, 0.0)
+        side["per_ik"] = {"unit": "energy points/s", "segments": few, "modelled_energy_seconds": t1,
+                          "value_without_fixed_stages": 1.0 / t1}
+        log(f"per-ik: {t1:.1f} s per energy (without the fixed stages)")
+    full = None
+    if args.full_scan:
         t0 = time.time()
-        S.run(ks_all[nk // 2: nk // 2 + 1], batch=1, seg_len=args.seg_len, groups=1)
-        torch.cuda.synchronize()
-        t1 = time.time() - t0
-        line["per_ik"] = {"value": 1.0 / t1, "unit": "energy points/s", "seconds": t1,
-                          "batched_over_per_ik": value * t1}
+        res = S.run(ks, batch=B, groups=1)
+        t_full = time.time() - t0
+        if not all(np.all(np.isfinite(r["D"])) for r in res):
+            raise SystemExit("non-finite dipole in the full scan")
+        full = {"seconds": t_full, "value": B / t_full, "unit": "energy points/s"}
+        log(f"full scan of the batch: {t_full:.1f} s ({B / t_full:.4f} energy points/s)")
+    S.close()   # the side runs build their own device tables
+
+    t_fixed, fixed_info = 0.0, None
+    if not args.no_side:
+        t_fixed, fixed_info = fixed_stage_seconds(p, ks, B)
+        log(f"fixed per-batch stages {t_fixed:.2f} s ({fixed_info})")
+    t_batch, model = batch_seconds(lay, N, pts, tf, tr, tb, t_fixed)
+    value = B / t_batch
+    log(f"modelled batch {t_batch:.1f} s -> {value:.4f} energy points/s")
+    if "per_ik" in side:
+        tb0, _ = batch_seconds(lay, N, pts, tf, tr, tb, 0.0)
+        side["per_ik"]["batched_over_per_ik"] = (B / tb0) * side["per_ik"]["modelled_energy_seconds"]
+
+    mflop = model_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
+                                   info["n_transition"], info["n_start"], p.n_occ)
+    if not args.no_side:
+        # the forward-mode derivative on the l_cont = 20 polarisation
+        # configuration: the same segment cycles with and without the tangent
+        pc, kc = synth.config("h2o_l20_pol")
+        S = sc.Scan(pc)
+        lay2 = S.layout()
+        kb = kc[(np.arange(B) * 37) % len(kc)]
+        few2 = spread_segments(lay2["n_seg"], 4)
+        ff, rf, bf, pf_, _ = cycle_rate(S, pc, lay2, kb, few2, warm=[few2[1]])
+        ft, rt, bt, pt_, _ = cycle_rate(S, pc, lay2, kb, few2, warm=[few2[1]], tangent=(1.0, 0.0))
+        S.close()
+        tb_f, _ = batch_seconds(lay2, pc.Nr - 1, pf_, ff, rf, bf, 0.0)
+        tb_t, _ = batch_seconds(lay2, pc.Nr - 1, pt_, ft, rt, bt, 0.0)
+        side["tangent"] = {"workload": f"h2o_l20_pol (l_cont={pc.l_cont}, N_tot={pc.n_tot}, Nr={pc.Nr}), batch {B}, "
+                                       f"segments {few2}, without the fixed stages",
+                           "forward_energies_per_s": B / tb_f, "with_tangent_energies_per_s": B / tb_t,
+                           "tangent_over_forward_cost": tb_t / tb_f}
+        log(f"tangent / forward cost {tb_t / tb_f:.2f} on h2o_l20_pol")
+
+    workload = (f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), Nr={p.Nr}, dr={p.dr}, "
+                f"k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
+                + ("closed channels allowed" if p.allow_closed else "all channels open")
+                + f"; batch of {B} energies.  A step is one full-size segment cycle: the batch runs {lay['L']} "
+                  f"radial points of forward ratio steps, their recompute and their back-propagation at segment s "
+                  f"(timed segments spread over the {lay['n_seg']} segments of the grid).  value = {B} / T_batch, "
+                  f"T_batch = (Nr-1) x the measured per-point cost of the three sweeps + the fit pre-pass (same "
+                  f"per-point costs) + the per-batch fixed stages (timed on a short-grid scan); a 256-energy scan "
+                  f"takes 256 / value seconds")
+    line = {
+        "metric": METRIC,
+        "value": value, "unit": "energy points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / max(args.steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "batch_energies": B, "segment_points": lay["L"], "n_segments": lay["n_seg"],
+                   "timed_segments": segs, "tiles": args.tiles},
+        "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS,
+                     "traffic": TOP_UPDATE_TRAFFIC.get(B, (None,))[0] if args.config == "h2o_l30" else None,
+                     "traffic_note": "HBM bytes of one 783x1566x783 Gauss-Jordan update launch at this batch, from "
+                                     "a calibrated rocprofv3 --pmc run (external calibration, not this process)",
+                     "kernel": "k_dgemm", "sampled_launches": int(prof[2]), "avg_launch_us": avg_launch_us,
+                     "event_pair_overhead_us": prof[15] * 1e3},
+        "model": model, "fixed_stages": fixed_info,
+        "energy_radial_steps_per_s": B * float(np.sum(pts)) / float(np.sum(tf) + np.sum(tr) + np.sum(tb)),
+        "model_tflop_per_energy": mflop / 1e12,
+        "path_tflops": mflop * value / 1e12,
+        "path_frac_of_fp64_peak": mflop * value / 1e12 / PEAK_FP64_MFMA_TFLOPS,
+        "setup_seconds": t_setup, "sampled_kernels": buckets, "prof_every": args.prof_every,
+    }
+    if full is not None:
+        line["full_scan"] = full
+    if side:
+        line["side"] = side
     if tune is not None:
         line["autotune"] = tune
-    S.close()   # the side measurements build their own device tables
-    if args.side_rmax > 0 and args.rmax == 0:
-        line["side"] = side_measurements(args, p, ks_all, B)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, batch_k(0), args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(p, ks, args.cpu_steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 

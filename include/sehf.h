@@ -166,6 +166,23 @@ typedef struct {
  * energy i; the same k may appear twice with two directions. */
 int  sehf_scan_run_tangent(sehf_scan* s, const double* k, const double* dir, int nk, int batch, int seg_len,
                            sehf_scan_out* out, sehf_scan_tangent_out* tout);
+/* One segment cycle of the scan at full size, for timing: the nk energies
+ * (one batch) run the forward ratio steps of radial segment `seg` (points
+ * [seg*L, min((seg+1)*L, Nr-1)), L = seg_len or ceil(sqrt(Nr-1)), with the
+ * Schur inverses one step ahead on the side stream), that segment's
+ * recompute from its start matrix, and its back-propagation with the fused
+ * dipole and fit work -- the per-segment work of sehf_scan_run, whose
+ * energies go through all segments once in each of the three sweeps.  The
+ * forward part and the recompute start from the analytic start matrix, the
+ * back-propagation from Z = [I; 0]; nothing is returned but the timing.
+ * With dir (as in sehf_scan_run_tangent) the tangent runs alongside.
+ * seconds[4] = host tables, forward part, recompute, back-propagation (may be
+ * NULL); layout[4] = L, number of segments, first point of the fit window,
+ * first segment of the fit pre-pass (may be NULL).  seg < 0 only fills
+ * layout.  Replaces no reference interface (the reference runs whole
+ * energies). */
+int  sehf_scan_run_segment(sehf_scan* s, const double* k, const double* dir, int nk, int seg, int seg_len,
+                           double* seconds, int* layout);
 /* Autotuner of the batch width, the number of concurrent groups and the
  * GEMM tile policy.  For each candidate (batch in 1, 2, 4, ... 128 with
  * groups * batch <= nk, groups in 1..2) that fits in device memory, the batch

@@ -196,11 +196,28 @@ static Mat inverse_gj(const Mat& A) {
 }
 // Rounding-order probes (noise-floor measurements only): bit 0 selects
 // Gauss-Jordan in place of LU for the dense inverses, bit 1 accumulates the
-// asymptotic-fit sums over the window from its outer end inwards.
+// asymptotic-fit sums over the window from its outer end inwards; bits 2, 3
+// and 4 select Gauss-Jordan for one stage's inverses only (Schur inverses,
+// forward ratio steps, K matrix); bit 5 runs the partial-pivot LU in 80-bit
+// extended precision (rounded to FP64 once at the end).
 static int g_inverse_mode = 0;
-static Mat inverse_general(const Mat& A) { return (g_inverse_mode & 1) ? inverse_gj(A) : inverse_pp(A); }
+enum InvStage { INV_ANY = -1, INV_SINV = 0, INV_FRP = 1, INV_KME = 2 };
+static Mat inverse_ext(const Mat& A) {
+    MatT<long double> a(A.r, A.c);
+    for (size_t t = 0; t < A.a.size(); ++t) a.a[t] = A.a[t];
+    const MatT<long double> x = inverse_pp(a);
+    Mat X(A.r, A.c);
+    for (size_t t = 0; t < X.a.size(); ++t) X.a[t] = static_cast<double>(x.a[t]);
+    return X;
+}
+static Mat inverse_general(const Mat& A, int stage = INV_ANY) {
+    const bool gj = (g_inverse_mode & 1) || (stage >= 0 && ((g_inverse_mode >> (2 + stage)) & 1));
+    if (gj) return inverse_gj(A);
+    if (g_inverse_mode & 32) return inverse_ext(A);
+    return inverse_pp(A);
+}
 // inverse_symmetric_indefinite, no-MKL branch (LapackInverse.hpp:174-176).
-static Mat inverse_symmetric_indefinite(const Mat& A) { Mat X = inverse_general(A); symmetrize(X); return X; }
+static Mat inverse_symmetric_indefinite(const Mat& A) { Mat X = inverse_general(A, INV_SINV); symmetrize(X); return X; }
 
 // Full-pivot LU inverse (Eigen FullPivLU; AsymptoticAmplitudes.cpp:317-322).
 static bool inverse_fullpiv(const Mat& A, Mat& Xout) {
@@ -983,7 +1000,7 @@ public:
         for (int j = 0; j < N_tot; ++j)
             for (int i = 0; i < N_tot; ++i)
                 R(i, j) = (12.0 * Y(i, j) - 10.0 * (i == j ? 1.0 : 0.0)) - Rprev(i, j);
-        Mat Ri = inverse_general(R);
+        Mat Ri = inverse_general(R, INV_FRP);
         symmetrize(Ri);
         return Ri;
     }
@@ -1052,7 +1069,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     Mat Kk, dKk;
     double unit_err_kme = 0.0;
     {
-        const Mat R = inverse_general(rinv[nM]);
+        const Mat R = inverse_general(rinv[nM], INV_KME);
         Mat Ssch(Np, Np);
         double npf = 0.0, nfp = 0.0;
         for (int j = Np; j < Nt; ++j) for (int i = 0; i < Np; ++i) npf += R(i, j) * R(i, j);
@@ -1064,7 +1081,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             for (int j = 0; j < Nf; ++j) for (int i = 0; i < Nf; ++i) Rff(i, j) = R(Np + i, Np + j);
             for (int j = 0; j < Nf; ++j) for (int i = 0; i < Np; ++i) Rpf(i, j) = R(i, Np + j);
             for (int j = 0; j < Np; ++j) for (int i = 0; i < Nf; ++i) Rfp(i, j) = R(Np + i, j);
-            const Mat Rffi = inverse_general(Rff);
+            const Mat Rffi = inverse_general(Rff, INV_KME);
             const Mat T = mul(Rpf, Rffi);
             gemm(-1.0, T, Rfp, 1.0, Ssch);
         }

@@ -20,6 +20,14 @@ void check_gj_info(const GjBuffers& gb, int batch, hipStream_t s, const char* wh
     std::vector<int> info(batch);
     SEHF_HIP_CHECK(hipMemcpyAsync(info.data(), gb.info.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
     SEHF_HIP_CHECK(hipStreamSynchronize(s));
+    // the flags are sticky between checks: clear them once read, so a later
+    // call on the same buffers reports only its own zero pivots
+    bool any = false;
+    for (int b = 0; b < batch; ++b) any = any || info[b] != 0;
+    if (any) {
+        fill_int(gb.info.p, static_cast<size_t>(batch), 0, s);
+        SEHF_HIP_CHECK(hipStreamSynchronize(s));
+    }
     for (int b = 0; b < batch; ++b)
         if (info[b] != 0)
             throw std::runtime_error(std::string(what) + ": singular matrix (zero pivot at index " +

@@ -401,6 +401,18 @@ double wall() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Fit window [n_fs, n_fe] (AsymptoticAmplitudes.cpp:70-91), clamped to the
+// 300-point tail that the driver keeps (main.cpp:1335).
+void fit_window(int Nr, int& n_fs, int& n_fe) {
+    const int N = Nr - 1;
+    const int asym_off = std::max(0, N - 300 + 1);
+    n_fs = std::max(Nr - 200, static_cast<int>(0.9 * Nr));
+    n_fe = Nr - 1 - 3;
+    n_fs = std::max(n_fs, asym_off);
+    n_fe = std::min(n_fe, asym_off + 300 - 1);
+    if (n_fe - n_fs + 1 < 10) throw std::runtime_error("asymptotic fit window is too small");
+}
+
 // One batch of energies through the whole path.  With `dir` (per energy the
 // direction (t_E, t_alpha)) the forward-mode derivative of every output along
 // t_E d/dE + t_alpha d/dalpha is propagated alongside (tout receives it).
@@ -428,8 +440,14 @@ struct ProbeSync {
 // inverses + ratio steps, with the tangent when dir is set) and return;
 // nothing is written to out.  g.t_probe = end of this group's steps minus the
 // common start of all groups (sync).
+// seg >= 0: one segment cycle (see sehf_scan_run_segment): the forward ratio
+// steps of radial segment `seg`, its recompute from the segment's start
+// matrix and its back-propagation, with nothing written to out; seg_t[4]
+// receives the seconds of the host tables, forward, recompute and
+// back-propagation parts.
 void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int nb, int L, sehf_scan_out* out,
-               sehf_scan_tangent_out* tout, int out_off, int probe = 0, ProbeSync* sync = nullptr) {
+               sehf_scan_tangent_out* tout, int out_off, int probe = 0, ProbeSync* sync = nullptr, int seg = -1,
+               double* seg_t = nullptr) {
     const double tw0 = wall();
     Setup& S = *s->S;
     hipStream_t st = g.st;
@@ -491,13 +509,8 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             q[mu] = ric_j(l, k * rN); q[Np + mu] = ric_y(l, k * rN);
         }
     }
-    // fit window (AsymptoticAmplitudes.cpp:70-91) clamped to the 300-point tail
-    const int asym_off = std::max(0, N - 300 + 1);
-    int n_fs = std::max(Nr - 200, static_cast<int>(0.9 * Nr));
-    int n_fe = Nr - 1 - 3;
-    n_fs = std::max(n_fs, asym_off);
-    n_fe = std::min(n_fe, asym_off + 300 - 1);
-    if (n_fe - n_fs + 1 < 10) throw std::runtime_error("asymptotic fit window is too small");
+    int n_fs = 0, n_fe = 0;
+    fit_window(Nr, n_fs, n_fe);
     const int n_fit = n_fe - n_fs + 1;
     std::vector<double> Jc(static_cast<size_t>(n_fit) * Np * nb), Yc(Jc.size()), coef(static_cast<size_t>(5) * Np * nb);
     std::vector<double> dJc(tan ? Jc.size() : 0), dYc(dJc.size()), dcoef(tan ? static_cast<size_t>(4) * Np * nb : 0);
@@ -650,6 +663,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     hipStream_t st2 = g.st2;
     SEHF_HIP_CHECK(hipEventRecord(g.ev_in, st));
     SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_in, 0));
+    // zero-pivot flags start clear for this batch (a failed earlier run on
+    // the same buffers may have left them set)
+    fill_int(b.gj.info.p, static_cast<size_t>(nb), 0, st);
+    fill_int(b.gj2.info.p, static_cast<size_t>(nb), 0, st2);
     const int64_t sJc = static_cast<int64_t>(n_fit) * Np;
 
     auto gj = [&](double* mat, double* outp, int n, bool sym) {
@@ -737,15 +754,20 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // Sinv_n goes to ring slot n & 1, one step ahead of its consumer: the
     // side stream waits until FRP(n-2) has read the slot; FRP(n) waits for
     // Sinv_n.
+    // A segment cycle's forward part starts at the segment's first point from
+    // the analytic start matrix (a ratio matrix of the right size and
+    // conditioning; the cycle's results are not kept).
+    const int n_first = seg >= 0 ? std::max(ns0, seg * L) : ns0;
     auto sinv_ahead = [&](int n) {
         const int slot = n & 1;
-        if (n - 2 >= ns0) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
+        if (n - 2 >= n_first) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
         sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
     };
-    const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1) : Nr - 2;
-    if (ns0 <= n_last) sinv_ahead(ns0);
-    for (int n = ns0; n <= n_last; ++n) {
+    const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
+                                 : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
+    if (n_first <= n_last) sinv_ahead(n_first);
+    for (int n = n_first; n <= n_last; ++n) {
         if (n + 1 <= n_last) sinv_ahead(n + 1);
         const int slot = n & 1;
         SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
@@ -776,7 +798,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     check_gj_info(b.gj, nb, st, "scan forward sweep");
     check_gj_info(b.gj2, nb, st2, "scan forward sweep (Schur inverses)");
     const double tw2 = wall();
+    if (seg_t) { seg_t[0] = tw1 - tw0; seg_t[1] = tw2 - tw1; }
     // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
+    if (seg < 0) {
     SEHF_HIP_CHECK(hipMemcpyAsync(b.Rc.p, b.Rp.p, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
     gj(b.Rc.p, b.U.p, Nt, false);   // U <- R = Rinv_{N-1}^{-1}
     get_block(b.Am.p, b.U.p, Nt, tt, 0, 0, Np, Np, nb, st);   // S_schur starts as R_psi_psi
@@ -847,6 +871,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     }
     // ---------------- outer boundary (BackPropagator.cpp:128-197, :538-551) ----------------
     psi_boundary(b.P.p, b.Kd.p, b.jy.p, Np, nb, st);
+    }   // seg < 0
     // Z_psi = psi + h^2/6 (E psi - V_N psi) + B_N f_N with f_N = -D_N^{-1} B_N^T psi, Z_f = 0
     auto zmap = [&](const double* Pin, double* Zout) {
         Gemm g;   // Vpsi = V_N psi_N
@@ -871,8 +896,8 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         }
         z_boundary(Zout, Pin, b.Vpsi.p, b.Bf.p, b.cE.p, s->c6, Np, Nt, nb, st);
     };
-    zmap(b.P.p, b.Z.p);
-    if (tan) {
+    if (seg < 0) zmap(b.P.p, b.Z.p);
+    if (tan && seg < 0) {
         // d psi_N from dK and the tables; dZ = zmap(d psi_N) + h^2/6 w_N psi_N
         dpsi_boundary(b.dP.p, b.Kd.p, b.dKd.p, b.jy.p, b.djy.p, Np, nb, st);
         zmap(b.dP.p, b.dZ.p);
@@ -894,7 +919,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         const size_t o = static_cast<size_t>(n - n_fs) * Np;
         fit_accum_t(b.dYJ.p, b.dYY.p, psi, dpsi, b.Jc.p + o, b.Yc.p + o, b.dJc.p + o, b.dYc.p + o, sJc, Np, nb, st);
     };
-    if (N < s->n_pts) dip_accum(N, b.P.p, b.acc.p);
+    if (N < s->n_pts && seg < 0) dip_accum(N, b.P.p, b.acc.p);
     // The fit window [n_fs, n_fe] ends 3 points below the boundary N.  Its
     // sums YJ, YY come from a pre-pass over the top segments (below), so the
     // main pass knows A and B and takes the fit residual point by point; no
@@ -912,8 +937,8 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         fill(b.dacc.p, static_cast<size_t>(ns) * Np * nb, 0.0, st);
         fill(b.dYJ.p, pp * nb, 0.0, st);
         fill(b.dYY.p, pp * nb, 0.0, st);
-        if (N < s->n_pts) dip_accum(N, b.dP.p, b.dacc.p);
-        if (N >= n_fs && N <= n_fe) fit_tangent(N, b.P.p, b.dP.p);
+        if (N < s->n_pts && seg < 0) dip_accum(N, b.dP.p, b.dacc.p);
+        if (N >= n_fs && N <= n_fe && seg < 0) fit_tangent(N, b.P.p, b.dP.p);
     }
     fill(b.red.p, 2 * nb, 0.0, st);
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
@@ -949,10 +974,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // sweep(main = false): the fit pre-pass, segments from the top down to the
     // one holding n_fs, psi only, stopping at n_fs.  sweep(main = true): the
     // whole grid with the dipole sums, the fit residual and the tangent.
+    // A segment cycle runs the main pass over its one segment (sg_hi = sg_lo)
+    // and, with t_rec set, records when the segment's recompute has finished.
     bool seg_pending = false;   // a previous segment's backward products may still read segS
-    auto sweep = [&](bool main) {
-        const int sg_lo = main ? 0 : n_fs / L;
-        for (int sg = n_seg - 1; sg >= sg_lo; --sg) {
+    double t_rec = 0.0;
+    auto sweep = [&](bool main, int sg_hi, int sg_lo, bool time_rec) {
+        for (int sg = sg_hi; sg >= sg_lo; --sg) {
             const int n0 = sg * L, n1 = std::min(n0 + L, N);
             const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
             const bool tg = main && tan;
@@ -985,6 +1012,11 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 }
                 prev = Rn;
                 dprev = dRn;
+            }
+            if (time_rec) {
+                SEHF_HIP_CHECK(hipStreamSynchronize(st2));
+                SEHF_HIP_CHECK(hipStreamSynchronize(st));
+                t_rec = wall();
             }
             progress(main ? "back-propagation segment" : "fit pre-pass segment", sg, n_seg);
             const int n_stop = main ? n0 : std::max(n0, n_fs);
@@ -1041,12 +1073,42 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             seg_pending = true;
         }
     };
-    sweep(false);
+    if (seg >= 0) {
+        // Segment cycle: the recompute starts from the same start matrix as
+        // the forward part, and the back-propagation from Z = [I; 0] (the
+        // boundary of a whole scan needs its K matrix); the fit residual uses
+        // A = I, B = 0.  The products, inverses and fused dipole/fit work are
+        // those of the scan's main pass over this segment.
+        const int64_t sZs = static_cast<int64_t>(Nt) * Np;
+        if (seg > 0) {
+            double* c0 = b.ckpt.p + static_cast<size_t>(seg) * nb * tt;
+            if (ns0 >= 2) diag_matrix(c0, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
+            else fill(c0, tt * nb, 0.0, st);
+            if (tan) fill(b.dckpt.p + static_cast<size_t>(seg) * nb * tt, tt * nb, 0.0, st);
+        }
+        fill(b.Z.p, sZs * nb, 0.0, st);
+        axpbyi(b.Z.p, Nt, sZs, b.Z.p, Nt, sZs, 0.0, nullptr, 0, 0, 0.0, 1.0, Np, Np, nb, st);
+        if (tan) fill(b.dZ.p, sZs * nb, 0.0, st);
+        fill(b.Am.p, pp * nb, 0.0, st);
+        axpbyi(b.Am.p, Np, pp, b.Am.p, Np, pp, 0.0, nullptr, 0, 0, 0.0, 1.0, Np, Np, nb, st);
+        fill(b.Bm.p, pp * nb, 0.0, st);
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        const double tb0 = wall();
+        sweep(true, seg, seg, true);
+        SEHF_HIP_CHECK(hipStreamSynchronize(st2));
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        check_gj_info(b.gj, nb, st, "segment cycle");
+        check_gj_info(b.gj2, nb, st2, "segment cycle (Schur inverses)");
+        const double tb1 = wall();
+        if (seg_t) { seg_t[2] = t_rec - tb0; seg_t[3] = tb1 - t_rec; }
+        return;
+    }
+    sweep(false, n_seg - 1, n_fs / L, false);
     // A, B of the fit from the completed window sums; the main pass then
     // starts again from the outer boundary
     fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
     zmap(b.P.p, b.Z.p);
-    sweep(true);
+    sweep(true, n_seg - 1, 0, false);
     check_gj_info(b.gj, nb, st, "scan back-propagation");
     check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
@@ -1274,9 +1336,45 @@ void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batc
         if (!errs[gi].empty()) throw std::runtime_error(errs[gi]);
 }
 
+// One segment cycle of the scan for all nk energies in one batch (group 0).
+void scan_segment(sehf_scan* s, const double* k, const double* dir, int nk, int seg, int seg_len, double* t,
+                  int* layout) {
+    const int N = s->Nr - 1;
+    int L = seg_len > 0 ? seg_len : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
+    L = std::max(L, s->n_start + 1);
+    const int n_seg = (N + L - 1) / L;
+    int n_fs = 0, n_fe = 0;
+    fit_window(s->Nr, n_fs, n_fe);
+    if (layout) { layout[0] = L; layout[1] = n_seg; layout[2] = n_fs; layout[3] = n_fs / L; }
+    if (seg < 0) return;   // layout query
+    if (nk <= 0 || !k) throw std::runtime_error("sehf_scan_run_segment: empty k list");
+    for (int i = 0; i < nk; ++i)
+        if (!(k[i] > 0.0)) throw std::runtime_error("sehf_scan_run_segment: every k must be > 0");
+    if (dir)
+        for (int i = 0; i < 2 * nk; ++i)
+            if (!std::isfinite(dir[i])) throw std::runtime_error("sehf_scan_run_segment: directions must be finite");
+    if (seg >= n_seg)
+        throw std::runtime_error("sehf_scan_run_segment: segment " + std::to_string(seg) + " outside [0, " +
+                                 std::to_string(n_seg) + ")");
+    if (s->groups.empty()) {
+        s->groups.emplace_back(new Group());
+        s->groups.back()->init();
+    }
+    SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
+    alloc_batch(s, *s->groups[0], nk, L, n_seg, 200, dir != nullptr);
+    double tt[4] = {0, 0, 0, 0};
+    run_batch(s, *s->groups[0], k, dir, nk, L, nullptr, nullptr, 0, 0, nullptr, seg, tt);
+    if (t) std::copy(tt, tt + 4, t);
+}
+
 }  // namespace
 
 extern "C" {
+
+int sehf_scan_run_segment(sehf_scan* s, const double* k, const double* dir, int nk, int seg, int seg_len,
+                          double* seconds, int* layout) {
+    return guarded([&] { scan_segment(s, k, dir, nk, seg, seg_len, seconds, layout); });
+}
 
 int sehf_scan_run(sehf_scan* s, const double* k, int nk, int batch, int seg_len, sehf_scan_out* out) {
     return guarded([&] { scan_run(s, k, nullptr, nk, batch, seg_len, out, nullptr); });

@@ -44,6 +44,7 @@ SIGNATURES = [
     ("sehf_scan_info", ctypes.c_int, [_vp, _ip]),
     ("sehf_scan_run", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     ("sehf_scan_run_tangent", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("sehf_scan_run_segment", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip]),
     ("sehf_scan_autotune", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp,
                                           _ip]),
     ("sehf_set_gemm_tiles", ctypes.c_int, [ctypes.c_int]),

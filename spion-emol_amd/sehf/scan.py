@@ -134,6 +134,30 @@ class Scan:
             res.append(r)
         return res
 
+    def layout(self, seg_len: int = 0) -> dict:
+        """Segment layout of the radial grid: L points per segment, the number
+        of segments, the fit window's first point and the first segment of
+        the fit pre-pass."""
+        lay = (ctypes.c_int * 4)()
+        check(load().sehf_scan_run_segment(self._h, None, None, 0, -1, int(seg_len), None, lay),
+              "sehf_scan_run_segment")
+        return dict(zip(("L", "n_seg", "n_fit_lo", "prepass_seg_lo"), list(lay)))
+
+    def run_segment(self, ks, seg: int, seg_len: int = 0, tangent=None) -> dict:
+        """One segment cycle (forward part, recompute, back-propagation of
+        radial segment `seg`) for all ks in one batch; returns its timing
+        (sehf_scan_run_segment)."""
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        nk = len(ks)
+        d = None
+        if tangent is not None:
+            d = np.asarray(tangent, dtype=np.float64)
+            d = np.ascontiguousarray(np.broadcast_to(d, (nk, 2)) if d.ndim == 1 else d)
+        t = (ctypes.c_double * 4)()
+        check(load().sehf_scan_run_segment(self._h, ks.ctypes.data_as(_dp), d.ctypes.data_as(_dp) if d is not None
+                                           else None, nk, int(seg), int(seg_len), t, None), "sehf_scan_run_segment")
+        return dict(zip(("host_tables", "forward", "recompute", "back"), list(t)))
+
     def autotune(self, ks, probe_steps: int = 8, tangent: bool = False) -> dict:
         """Pick the batch width, group count and GEMM tile policy for these
         energies by timing probe_steps radial steps per candidate
