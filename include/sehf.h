@@ -82,6 +82,22 @@ int  sehf_frp_step(sehf_frp* s, const double* W_inv, double* Rinv_out, double* s
  * HIP backend always takes the LU-equivalent path, so the request is
  * recorded and is_symmetric_inverse_active() reports 0. */
 int  sehf_frp_enable_symmetric_inverse(sehf_frp* s, int enable);
+/* Forward step without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Runs the
+ * kernels of sehf_frp_step on caller-owned DEVICE buffers (column-major
+ * N_total^2 FP64 each): Rinv_out = 1/2 (X + X^T), X = (12 W_inv - 10 I -
+ * R_prev)^{-1}; no handle, no state.  The workspace is a fixed static device
+ * array (orders up to SEHF_FRP_NOALLOC_MAX_N, about 39 MiB held on each
+ * device the library runs on until the process ends); larger orders are
+ * rejected.  The call runs on the default stream and returns after it has
+ * finished (inputs must be complete on the device before the call).  All
+ * calls on one device share one workspace: calls that overlap on one device
+ * race and must be serialised by the caller.  sym_err (host, may be NULL)
+ * receives max|Rinv - Rinv^T|. */
+#define SEHF_FRP_NOALLOC_MAX_N 1600
+int  sehf_frp_step_noalloc(int N_total, const double* W_inv, const double* R_prev, double* Rinv_out,
+                           double* sym_err);
 int  sehf_frp_is_symmetric_inverse_active(const sehf_frp* s);
 void sehf_frp_destroy(sehf_frp* s);
 

@@ -26,6 +26,7 @@ SIGNATURES = [
     ("sehf_frp_init_R_prev_inv", ctypes.c_int, [_vp, _dp]),
     ("sehf_frp_step", ctypes.c_int, [_vp, _dp, _dp, _dp]),
     ("sehf_frp_enable_symmetric_inverse", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("sehf_frp_step_noalloc", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _dp]),
     ("sehf_frp_is_symmetric_inverse_active", ctypes.c_int, [_vp]),
     ("sehf_frp_destroy", None, [_vp]),
     ("sehf_sinv_create", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
