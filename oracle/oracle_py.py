@@ -230,3 +230,9 @@ def five_point_derivative(f, dr):
 def simpson(f, h):
     f = np.ascontiguousarray(f, dtype=np.float64)
     return lib().sehf_oracle_simpson(_ptr(f), f.size, h)
+
+
+def set_threads(n: int):
+    """Threads for the loops over independent columns inside one solve
+    (bitwise the same results as one thread)."""
+    lib().sehf_oracle_set_threads(int(n))

@@ -76,23 +76,46 @@ static Mat transpose(const Mat& A) {
 // product is formed first and then combined with C, so an update such as
 // S = A - Bs*B^T groups as A_ij - (sum_p ...), the way the reference's
 // GEMM evaluates it (SchurInverter.cpp:347-351).
+// Rounding-order probe (noise-floor measurements only): with g_gemm_rev set
+// on this thread the product sums run over p in reverse order.
+static thread_local bool g_gemm_rev = false;
+// Threads for the loops over independent output columns (GEMM, LU update,
+// LU solve): each column's operations are unchanged, so the results are
+// bitwise those of one thread.
+static int g_threads = 1;
 static void gemm(double alpha, const Mat& A, const Mat& B, double beta, Mat& C) {
     const int m = A.r, k = A.c, n = B.c;
     if (C.r != m || C.c != n) { C.resize(m, n); beta = 0.0; }
-    std::vector<double> t(m);
-    for (int j = 0; j < n; ++j) {
-        double* cj = C.col(j);
+    // output columns in groups of 8 that share each pass over A's columns;
+    // every element keeps its own sum in p order
+    constexpr int SG = 8;
+    const int ng = (n + SG - 1) / SG;
+    const bool par = g_threads > 1 && static_cast<double>(m) * k * n > 4e6;
+#pragma omp parallel num_threads(g_threads) if (par)
+    {
+    std::vector<double> t(static_cast<size_t>(m) * SG);
+#pragma omp for schedule(static)
+    for (int g = 0; g < ng; ++g) {
+        const int j0 = g * SG, gw = std::min(SG, n - j0);
         std::fill(t.begin(), t.end(), 0.0);
-        const double* bj = B.col(j);
-        for (int p = 0; p < k; ++p) {
-            const double s = bj[p];
-            if (s == 0.0) continue;
+        for (int pp = 0; pp < k; ++pp) {
+            const int p = g_gemm_rev ? k - 1 - pp : pp;
             const double* ap = A.col(p);
-            for (int i = 0; i < m; ++i) t[i] += s * ap[i];
+            for (int q = 0; q < gw; ++q) {
+                const double s = B(p, j0 + q);
+                if (s == 0.0) continue;
+                double* tq = t.data() + static_cast<size_t>(q) * m;
+                for (int i = 0; i < m; ++i) tq[i] += s * ap[i];
+            }
         }
-        if (beta == 0.0) for (int i = 0; i < m; ++i) cj[i] = alpha * t[i];
-        else if (beta == 1.0) for (int i = 0; i < m; ++i) cj[i] = cj[i] + alpha * t[i];
-        else for (int i = 0; i < m; ++i) cj[i] = beta * cj[i] + alpha * t[i];
+        for (int q = 0; q < gw; ++q) {
+            double* cj = C.col(j0 + q);
+            const double* tq = t.data() + static_cast<size_t>(q) * m;
+            if (beta == 0.0) for (int i = 0; i < m; ++i) cj[i] = alpha * tq[i];
+            else if (beta == 1.0) for (int i = 0; i < m; ++i) cj[i] = cj[i] + alpha * tq[i];
+            else for (int i = 0; i < m; ++i) cj[i] = beta * cj[i] + alpha * tq[i];
+        }
+    }
     }
 }
 static Mat mul(const Mat& A, const Mat& B) { Mat C(A.r, B.c); gemm(1.0, A, B, 0.0, C); return C; }
@@ -114,47 +137,90 @@ template <class T>
 static double absval(const T& v) { return std::abs(v); }
 
 // In-place partial-pivot LU: P A = L U; perm[k] = row swapped into k.
+// Right-looking elimination in the order of the unblocked algorithm: at
+// step k the pivot is the first row of largest |a(i,k)|, i >= k; rows k
+// and p swap in every column; l = a(i,k) / a(k,k); then a(i,j) -= l * a(k,j)
+// for i, j > k, skipped where a(k,j) == 0 (a zero pivot skips the step).
+// For cache reuse the columns right of a 32-column panel receive the
+// panel's steps together, one column at a time and step by step, after the
+// panel is factored: each element still sees the same operations in the
+// same order (the row swaps commute with the deferred updates because they
+// move the panel's multipliers with the rows), so the factors are bitwise
+// those of the unblocked loop.
 template <class T>
 static void lu_partial(MatT<T>& a, std::vector<int>& perm) {
     const int n = a.r;
     perm.resize(n);
-    for (int k = 0; k < n; ++k) {
-        int p = k; double best = absval(a(k, k));
-        for (int i = k + 1; i < n; ++i) { const double v = absval(a(i, k)); if (v > best) { best = v; p = i; } }
-        perm[k] = p;
-        if (p != k) for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
-        const T piv = a(k, k);
-        if (piv == T(0)) continue;
-        T* ck = a.col(k);
-        for (int i = k + 1; i < n; ++i) ck[i] /= piv;
-        for (int j = k + 1; j < n; ++j) {
+    constexpr int NB = 32;
+    std::vector<char> live(n, 0);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int k1 = std::min(n, k0 + NB);
+        for (int k = k0; k < k1; ++k) {
+            int p = k; double best = absval(a(k, k));
+            for (int i = k + 1; i < n; ++i) { const double v = absval(a(i, k)); if (v > best) { best = v; p = i; } }
+            perm[k] = p;
+            if (p != k) for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
+            const T piv = a(k, k);
+            if (piv == T(0)) continue;
+            live[k] = 1;
+            T* ck = a.col(k);
+            for (int i = k + 1; i < n; ++i) ck[i] /= piv;
+            for (int j = k + 1; j < k1; ++j) {
+                T* cj = a.col(j);
+                const T f = cj[k];
+                if (f == T(0)) continue;
+                for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * f;
+            }
+        }
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1 && n - k1 > 64)
+        for (int j = k1; j < n; ++j) {
             T* cj = a.col(j);
-            const T f = cj[k];
-            if (f == T(0)) continue;
-            for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * f;
+            for (int k = k0; k < k1; ++k) {
+                if (!live[k]) continue;
+                const T f = cj[k];
+                if (f == T(0)) continue;
+                const T* ck = a.col(k);
+                for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * f;
+            }
         }
     }
 }
-// Solve (L U) X = P B in place on X (B given in X), using LU from lu_partial.
+// Solve (L U) X = P B in place on X (B given in X), using LU from lu_partial:
+// per column, unit-lower forward substitution then upper backward
+// substitution, skipping zero entries.  Columns are taken in groups of 8
+// that share each pass over L and U; every column sees the same operations
+// in the same order as alone.
 template <class T>
 static void lu_solve(const MatT<T>& lu, const std::vector<int>& perm, MatT<T>& X) {
     const int n = lu.r, m = X.c;
     for (int k = 0; k < n; ++k)
         if (perm[k] != k) for (int j = 0; j < m; ++j) std::swap(X(k, j), X(perm[k], j));
-    for (int j = 0; j < m; ++j) {
-        T* x = X.col(j);
+    constexpr int SG = 8;
+    const int ng = (m + SG - 1) / SG;
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1 && static_cast<double>(n) * n * m > 4e6)
+    for (int g = 0; g < ng; ++g) {
+        const int j0 = g * SG, gw = std::min(SG, m - j0);
+        T* xs[SG];
+        for (int q = 0; q < gw; ++q) xs[q] = X.col(j0 + q);
         for (int k = 0; k < n; ++k) {            // unit-lower forward
-            const T xk = x[k];
-            if (xk == T(0)) continue;
             const T* lk = lu.col(k);
-            for (int i = k + 1; i < n; ++i) x[i] -= lk[i] * xk;
+            for (int q = 0; q < gw; ++q) {
+                T* x = xs[q];
+                const T xk = x[k];
+                if (xk == T(0)) continue;
+                for (int i = k + 1; i < n; ++i) x[i] -= lk[i] * xk;
+            }
         }
         for (int k = n - 1; k >= 0; --k) {       // upper backward
-            x[k] /= lu(k, k);
-            const T xk = x[k];
-            if (xk == T(0)) continue;
             const T* uk = lu.col(k);
-            for (int i = 0; i < k; ++i) x[i] -= uk[i] * xk;
+            const T d = lu(k, k);
+            for (int q = 0; q < gw; ++q) {
+                T* x = xs[q];
+                x[k] /= d;
+                const T xk = x[k];
+                if (xk == T(0)) continue;
+                for (int i = 0; i < k; ++i) x[i] -= uk[i] * xk;
+            }
         }
     }
 }
@@ -199,7 +265,9 @@ static Mat inverse_gj(const Mat& A) {
 // asymptotic-fit sums over the window from its outer end inwards; bits 2, 3
 // and 4 select Gauss-Jordan for one stage's inverses only (Schur inverses,
 // forward ratio steps, K matrix); bit 5 runs the partial-pivot LU in 80-bit
-// extended precision (rounded to FP64 once at the end).
+// extended precision (rounded to FP64 once at the end); bit 6 sums the
+// products of the W^{-1} application (forward ratio steps and
+// back-propagation) in reverse order.
 static int g_inverse_mode = 0;
 enum InvStage { INV_ANY = -1, INV_SINV = 0, INV_FRP = 1, INV_KME = 2 };
 static Mat inverse_ext(const Mat& A) {
@@ -911,6 +979,11 @@ public:
     // ---- WInverseOperator::apply (WInverseOperator.cpp:75-128) ----
     // Y = W_n^{-1} Z, with Sinv, B(n), D^{-1}(n).
     void winv_apply(int n, const Mat& Sinv, const Mat& Z, Mat& Y) const {
+        struct RevScope {   // bit 6 of the rounding-order probes: reversed sums here
+            bool old = g_gemm_rev;
+            RevScope() { if (g_inverse_mode & 64) g_gemm_rev = true; }
+            ~RevScope() { g_gemm_rev = old; }
+        } rev_scope;
         const int nc = Z.c;
         Y.resize(N_tot, nc);
         const std::vector<double> Di = Dinv(n);
@@ -1027,49 +1100,77 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
     const bool tan = out->tangent != 0;
     const double tE = out->tan_E, tA = out->tan_alpha, dk = tE / k;
     auto wn = [&](int n) { return tE - tA * S.pol_unit_at(n); };
-    // ---- Sinv (SchurInverter::build, SchurInverter.cpp:465-473) ----
-    std::vector<Mat> sinv(Nr);
+    // ---- Sinv (SchurInverter::build, SchurInverter.cpp:465-473) and FRP
+    // (ForwardRPropagator::run, ForwardRPropagator.cpp:136-401) in one sweep.
+    // The reference keeps every Sinv_n and Rinv_n (disk or RAM); here only
+    // Rinv at every L-th point is kept (L = ceil(sqrt(Nr-1)) > n_start) and
+    // each segment is recomputed by the same operations during the
+    // back-propagation below, so every value is bitwise the same as with
+    // the full store.
+    const int ns = S.n_start();
+    const int N = Nr - 1;
+    int L = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(N))));
+    L = std::max(L, ns + 1);
+    const int n_seg = (N + L - 1) / L;
+    std::vector<Mat> ck(n_seg), dck(tan ? n_seg : 0);   // ck[s] = Rinv_{sL-1} (s >= 1)
     std::vector<char> sig(Nr, 1);
     int shA = 0, shS = 0;
-    for (int n = 0; n < Nr; ++n) {
-        const int a0 = shA, s0 = shS;
-        sinv[n] = S.sinv_at(n, E, shA, shS);
-        sig[n] = (shA == a0 && shS == s0) ? 1 : 0;
-    }
-    const double t1 = now_s(); ts[1] = t1 - t0;
-    // ---- FRP (ForwardRPropagator::run, ForwardRPropagator.cpp:136-401) ----
-    const int ns = S.n_start();
-    std::vector<Mat> rinv(Nr), drinv(tan ? Nr : 0);
-    Mat Rprev(Nt, Nt);
-    for (int n = 0; n < ns; ++n) {
-        rinv[n] = S.analytic_init(n, E);
-        if (tan) drinv[n] = S.analytic_init_tangent(n, E, tE, tA);
-        if (n == ns - 1) Rprev = rinv[n];
-    }
     Mat Epsi(Nt, Np);
     for (int i = 0; i < Np; ++i) Epsi(i, i) = 1.0;
-    for (int n = ns; n < Nr; ++n) {
-        rinv[n] = S.frp_step(n, sinv[n], Rprev);
-        Rprev = rinv[n];
-        if (tan) {
-            // dRinv_n = Rinv_n (dRinv_{n-1} + 12 sig h^2/6 w_n G G^T) Rinv_n,
-            // G = W_n^{-1}[:, psi] (the derivative of 12 W^{-1} - 10 I - Rinv_{n-1})
-            Mat G;
-            S.winv_apply(n, sinv[n], Epsi, G);
-            Mat H = (n >= 1) ? drinv[n - 1] : Mat(Nt, Nt);
-            gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, transpose(G), 1.0, H);
-            const Mat T = mul(H, rinv[n]);
-            drinv[n] = mul(rinv[n], T);
-            symmetrize(drinv[n]);
+    auto dump_at = [](int nd, const int* idx, int n) {
+        for (int i = 0; i < nd; ++i) if (idx[i] == n) return i;
+        return -1;
+    };
+    auto put_at = [](double* dst, size_t stride, int i, const Mat& M) {
+        if (dst && i >= 0) std::memcpy(dst + static_cast<size_t>(i) * stride, M.a.data(), M.a.size() * sizeof(double));
+    };
+    // one ratio step's tangent: dRinv_n = Rinv_n (dRinv_{n-1} + 12 sig h^2/6 w_n G G^T) Rinv_n,
+    // G = W_n^{-1}[:, psi] (the derivative of 12 W^{-1} - 10 I - Rinv_{n-1})
+    auto frp_tan = [&](int n, const Mat& sv, const Mat& Rn, const Mat* dprev) {
+        Mat G;
+        S.winv_apply(n, sv, Epsi, G);
+        Mat H = dprev ? *dprev : Mat(Nt, Nt);
+        gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, transpose(G), 1.0, H);
+        const Mat T = mul(H, Rn);
+        Mat dR = mul(Rn, T);
+        symmetrize(dR);
+        return dR;
+    };
+    Mat Rprev(Nt, Nt), dRprev;
+    for (int n = 0; n < Nr; ++n) {
+        const int a0 = shA, s0 = shS;
+        const Mat sv = S.sinv_at(n, E, shA, shS);
+        sig[n] = (shA == a0 && shS == s0) ? 1 : 0;
+        put_at(out->dump_sinv, static_cast<size_t>(Np) * Np, dump_at(out->n_dump_sinv, out->dump_sinv_n, n), sv);
+        if (n > N - 1) continue;   // Rinv_N is not used
+        Mat Rn, dRn;
+        if (n < ns) {
+            Rn = S.analytic_init(n, E);
+            if (tan) dRn = S.analytic_init_tangent(n, E, tE, tA);
+        } else {
+            Rn = S.frp_step(n, sv, Rprev);
+            if (tan) dRn = frp_tan(n, sv, Rn, n >= 1 ? &dRprev : nullptr);
+        }
+        const int di = dump_at(out->n_dump_rinv, out->dump_rinv_n, n);
+        put_at(out->dump_rinv, static_cast<size_t>(Nt) * Nt, di, Rn);
+        if (tan) put_at(out->dump_drinv, static_cast<size_t>(Nt) * Nt, di, dRn);
+        Rprev = std::move(Rn);
+        if (tan) dRprev = std::move(dRn);
+        if ((n + 1) % L == 0 && (n + 1) / L < n_seg) {
+            ck[(n + 1) / L] = Rprev;
+            if (tan) dck[(n + 1) / L] = dRprev;
         }
     }
-    const double t2 = now_s(); ts[2] = t2 - t1;
+    const Mat& R_last = Rprev;    // Rinv_{N-1}
+    const Mat& dR_last = dRprev;
+    const double t1 = now_s(); ts[1] = t1 - t0;
+    const double t2 = t1; ts[2] = 0.0;
     // ---- KMatrixExtractor::extract (KMatrixExtractor.cpp:75-198) ----
     const int nM = Nr - 2;
     Mat Kk, dKk;
     double unit_err_kme = 0.0;
     {
-        const Mat R = inverse_general(rinv[nM], INV_KME);
+        const Mat R = inverse_general(R_last, INV_KME);
         Mat Ssch(Np, Np);
         double npf = 0.0, nfp = 0.0;
         for (int j = Np; j < Nt; ++j) for (int i = 0; i < Np; ++i) npf += R(i, j) * R(i, j);
@@ -1109,7 +1210,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             // S_schur = ((Rinv_{nM})_psi,psi)^{-1}, so dS = -S (dRinv)_psi,psi S;
             // dK = sym(Bm^{-1} (dAm - dBm K)).
             Mat dRpp(Np, Np);
-            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dRpp(i, j) = drinv[nM](i, j);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dRpp(i, j) = dR_last(i, j);
             Mat dSs(Np, Np);
             gemm(-1.0, mul(Ssch, dRpp), Ssch, 0.0, dSs);
             Mat dAm(Np, Np), dBm(Np, Np);
@@ -1164,19 +1265,120 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
         }
     }
     const double t3 = now_s(); ts[3] = t3 - t2;
-    // ---- BackPropagator::run (BackPropagator.cpp:199-758) ----
-    const int N = Nr - 1;
+    // ---- energy-independent dipole tables (DipoleMatrixElement.cpp:584-965),
+    // so the integrands can be formed point by point during the sweep ----
     const int n_main_hi = std::min(Nr - 1, std::max(1, S.n_tr - 1));   // main.cpp:1267-1270
-    const int n_asym = 300, asym_off = N - n_asym + 1;                  // main.cpp:1335
-    std::vector<Mat> psi_keep(n_main_hi + 1), psi_asym(n_asym);
-    std::vector<Mat> dpsi_keep(tan ? n_main_hi + 1 : 0), dpsi_asym(tan ? n_asym : 0);
+    // initial state and bystander orbitals (main.cpp:746-822)
+    const int Nlm_init = S.n_lambda;
+    const int n_chi = S.n_tr;
+    Mat chi_init(Nr, Nlm_init);
+    std::vector<Mat> phi(n_occ, Mat(Nr, Nlm_init));
+    for (int j = 0; j < n_occ; ++j)
+        for (int ir = 0; ir < n_chi; ++ir)
+            for (int lam = 0; lam < Nlm_init; ++lam) phi[j](ir, lam) = S.chi[ir](j, lam);
+    if (S.P.has_init) {
+        // /initial_state/psi_lm read raw (no r factor), main.cpp:781-802
+        const int Nr_take = std::min(n_chi, Nr);
+        for (int ir = 0; ir < Nr_take; ++ir)
+            for (int lam = 0; lam < Nlm_init; ++lam) chi_init(ir, lam) = S.P.init_psi_lm[static_cast<size_t>(lam) * Nr + ir];
+    } else {
+        for (int ir = 0; ir < n_chi; ++ir)
+            for (int lam = 0; lam < Nlm_init; ++lam) chi_init(ir, lam) = S.chi[ir](n_occ - 1, lam);
+    }
+    Mat dchi(Nr, Nlm_init);
+    {
+        std::vector<double> f(Nr), df(Nr);
+        for (int nu = 0; nu < Nlm_init; ++nu) {
+            for (int ir = 0; ir < Nr; ++ir) f[ir] = chi_init(ir, nu);
+            five_point_derivative(f.data(), Nr, h, df.data());
+            for (int ir = 0; ir < Nr; ++ir) dchi(ir, nu) = df[ir];
+        }
+    }
+    std::vector<int> l_nu(Nlm_init), m_nu(Nlm_init);
+    for (int nu = 0; nu < Nlm_init; ++nu) idx_to_lm(nu, l_nu[nu], m_nu[nu]);
+    auto velocity_coef = [](int lm, int ln) -> double {    // DipoleMatrixElement.cpp:90-95
+        if (lm == ln + 1) return -static_cast<double>(ln + 1);
+        if (lm == ln - 1) return static_cast<double>(ln);
+        return 0.0;
+    };
+    const int qs[3] = {+1, -1, 0};
+    std::vector<Mat> Ang(3, Mat(Np, Nlm_init));
+    for (int qi = 0; qi < 3; ++qi)
+        for (int mu = 0; mu < Np; ++mu)
+            for (int nu = 0; nu < Nlm_init; ++nu) {
+                if (std::abs(S.l_mu[mu] - l_nu[nu]) != 1) continue;
+                const double a = std::sqrt(4.0 * M_PI / 3.0) * gaunt_real(S.l_mu[mu], S.m_mu[mu], 1, qs[qi], l_nu[nu], m_nu[nu]);
+                if (a != 0.0) Ang[qi](mu, nu) = a;
+            }
+    const int n_hi = std::min(Nr, n_main_hi + 1), n_lo = 0, n_pts = n_hi - n_lo;
+    if (n_pts < 5) throw std::runtime_error("DipoleMatrixElement: too few radial points");
+    std::vector<Mat> Xi(6, Mat(n_pts, Np));
+    for (int slot = 0; slot < 6; ++slot) {
+        const Mat& Ag = Ang[slot % 3];
+        const bool L = slot < 3;
+        for (int ir = n_lo; ir < n_hi; ++ir) {
+            const double r = h + ir * h, inv_r = (r > 1e-30) ? 1.0 / r : 0.0;
+            for (int mu = 0; mu < Np; ++mu) {
+                double s = 0.0;
+                for (int nu = 0; nu < Nlm_init; ++nu) {
+                    const double a = Ag(mu, nu);
+                    if (a == 0.0) continue;
+                    const double c = chi_init(ir, nu);
+                    if (L) s += a * c;
+                    else s += a * (dchi(ir, nu) + velocity_coef(S.l_mu[mu], l_nu[nu]) * inv_r * c);
+                }
+                Xi[slot](ir - n_lo, mu) = s;
+            }
+        }
+    }
+    std::vector<Mat> integ(6, Mat(n_pts, Np));           // [slot](row, beta)
+    std::vector<Mat> integ_b(n_occ, Mat(n_pts, Np));     // [alpha](row, beta)
+    std::vector<Mat> dinteg(tan ? 6 : 0, Mat(n_pts, Np)), dinteg_b(tan ? n_occ : 0, Mat(n_pts, Np));
+    auto integrands = [&](int ir, const Mat& ps, const Mat* dps) {
+        if (ir < n_lo || ir >= n_hi) return;
+        const int row = ir - n_lo;
+        const double r = h + ir * h;
+        for (int slot = 0; slot < 6; ++slot) {
+            const double rf = (slot < 3) ? r : 1.0;
+            for (int b = 0; b < Np; ++b) {
+                double s = 0.0;
+                for (int mu = 0; mu < Np; ++mu) s += Xi[slot](row, mu) * ps(mu, b);
+                integ[slot](row, b) = rf * s;
+            }
+        }
+        for (int a = 0; a < n_occ; ++a)
+            for (int b = 0; b < Np; ++b) {
+                double s = 0.0;
+                for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * ps(mu, b);
+                integ_b[a](row, b) = s;
+            }
+        if (!dps) return;
+        for (int slot = 0; slot < 6; ++slot) {
+            const double rf = (slot < 3) ? r : 1.0;
+            for (int b = 0; b < Np; ++b) {
+                double s = 0.0;
+                for (int mu = 0; mu < Np; ++mu) s += Xi[slot](row, mu) * (*dps)(mu, b);
+                dinteg[slot](row, b) = rf * s;
+            }
+        }
+        for (int a = 0; a < n_occ; ++a)
+            for (int b = 0; b < Np; ++b) {
+                double s = 0.0;
+                for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * (*dps)(mu, b);
+                dinteg_b[a](row, b) = s;
+            }
+    };
+    // ---- BackPropagator::run (BackPropagator.cpp:199-758) ----
+    // The driver keeps psi on [0, n_tr-1] for the dipole (main.cpp:1267-1270),
+    // whose integrands are formed here point by point, and on the 300-point
+    // tail for the asymptotic fit (main.cpp:1335), which is stored.
+    const int n_asym = 300, asym_off = N - n_asym + 1;
+    std::vector<Mat> psi_asym(n_asym), dpsi_asym(tan ? n_asym : 0);
     auto store = [&](int n, const Mat& psi) {
         if (n >= asym_off && n <= N) psi_asym[n - asym_off] = psi;
-        if (n >= 0 && n <= n_main_hi) psi_keep[n] = psi;
     };
     auto dstore = [&](int n, const Mat& dpsi) {
         if (n >= asym_off && n <= N) dpsi_asym[n - asym_off] = dpsi;
-        if (n >= 0 && n <= n_main_hi) dpsi_keep[n] = dpsi;
     };
     store(N, psiN);
     if (tan) dstore(N, dpsiN);
@@ -1204,52 +1406,84 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
             for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dZ(i, j) += c * psiN(i, j);
         }
     }
+    integrands(N, psiN, tan ? &dpsiN : nullptr);
+    {
+        const int di = dump_at(out->n_dump_psi, out->dump_psi_n, N);
+        put_at(out->dump_psi, static_cast<size_t>(Np) * Np, di, psiN);
+        if (tan) put_at(out->dump_dpsi, static_cast<size_t>(Np) * Np, di, dpsiN);
+    }
     Mat Y, Yd;
-    for (int n = N - 1; n >= 0; --n) {
-        const Mat Zold = tan ? Z : Mat();
-        Z = mul(rinv[n], Z);
-        S.winv_apply(n, sinv[n], Z, Y);
-        Mat psi(Np, Np);
-        for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) psi(i, j) = Y(i, j);
-        store(n, psi);
-        if (tan) {
-            // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start,
-            // dRinv_n = Rinv_n H_n Rinv_n with H_n = dRinv_{n-1} + s_n G G^T,
-            // so dRinv_n Z_{n+1} = Rinv_n H_n Z_n and G^T Z_n = psi_n:
-            //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + s_n G psi_n).
-            // This grouping never multiplies by dRinv_n itself, which grows
-            // like 1/delta^2 where Rinv_n passes near a pole (a near-singular
-            // ratio); its rounding error then lies along the pole direction,
-            // which Rinv_{n+1} maps to O(delta).
-            Mat br = dZ;
-            if (n >= ns) {
-                gemm(1.0, drinv[n - 1], Z, 1.0, br);
-                Mat G;
-                S.winv_apply(n, sinv[n], Epsi, G);
-                gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, psi, 1.0, br);
-                dZ = mul(rinv[n], br);
+    std::vector<Mat> seg_s(L), seg_r(L), seg_dr(tan ? L : 0);
+    for (int sg = n_seg - 1; sg >= 0; --sg) {
+        const int n0 = sg * L, n1 = std::min(n0 + L, N);
+        // recompute the segment's Sinv, Rinv (and dRinv) from its checkpoint
+        for (int n = n0; n < n1; ++n) {
+            int dA = 0, dS = 0;
+            seg_s[n - n0] = S.sinv_at(n, E, dA, dS);
+            if (n < ns) {
+                seg_r[n - n0] = S.analytic_init(n, E);
+                if (tan) seg_dr[n - n0] = S.analytic_init_tangent(n, E, tE, tA);
             } else {
-                dZ = mul(rinv[n], br);
-                gemm(1.0, drinv[n], Zold, 1.0, dZ);
+                const Mat& prev = (n == n0) ? ck[sg] : seg_r[n - n0 - 1];
+                seg_r[n - n0] = S.frp_step(n, seg_s[n - n0], prev);
+                if (tan) {
+                    const Mat* dprev = (n == n0) ? &dck[sg] : &seg_dr[n - n0 - 1];
+                    seg_dr[n - n0] = frp_tan(n, seg_s[n - n0], seg_r[n - n0], n >= 1 ? dprev : nullptr);
+                }
             }
-            // dpsi = (W^{-1} dZ')_psi + (dW^{-1} Z')_psi, dW^{-1} = -sig h^2/6 w G G^T
-            // and G^T Z' = psi, G_psi = Sinv
-            S.winv_apply(n, sinv[n], dZ, Yd);
-            Mat dpsi(Np, Np);
-            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dpsi(i, j) = Yd(i, j);
-            gemm(-(sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), sinv[n], psi, 1.0, dpsi);
-            dstore(n, dpsi);
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const Mat& rn = seg_r[n - n0];
+            const Mat& sn = seg_s[n - n0];
+            const Mat Zold = tan ? Z : Mat();
+            Z = mul(rn, Z);
+            S.winv_apply(n, sn, Z, Y);
+            Mat psi(Np, Np);
+            for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) psi(i, j) = Y(i, j);
+            Mat dpsi;
+            if (tan) {
+                // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start,
+                // dRinv_n = Rinv_n H_n Rinv_n with H_n = dRinv_{n-1} + s_n G G^T,
+                // so dRinv_n Z_{n+1} = Rinv_n H_n Z_n and G^T Z_n = psi_n:
+                //   dZ_n = Rinv_n (dZ_{n+1} + dRinv_{n-1} Z_n + s_n G psi_n).
+                // This grouping never multiplies by dRinv_n itself, which grows
+                // like 1/delta^2 where Rinv_n passes near a pole (a near-singular
+                // ratio); its rounding error then lies along the pole direction,
+                // which Rinv_{n+1} maps to O(delta).
+                Mat br = dZ;
+                if (n >= ns) {
+                    const Mat& dRm1 = (n == n0) ? dck[sg] : seg_dr[n - n0 - 1];
+                    gemm(1.0, dRm1, Z, 1.0, br);
+                    Mat G;
+                    S.winv_apply(n, sn, Epsi, G);
+                    gemm(12.0 * (sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), G, psi, 1.0, br);
+                    dZ = mul(rn, br);
+                } else {
+                    dZ = mul(rn, br);
+                    gemm(1.0, seg_dr[n - n0], Zold, 1.0, dZ);
+                }
+                // dpsi = (W^{-1} dZ')_psi + (dW^{-1} Z')_psi, dW^{-1} = -sig h^2/6 w G G^T
+                // and G^T Z' = psi, G_psi = Sinv
+                S.winv_apply(n, sn, dZ, Yd);
+                dpsi.resize(Np, Np);
+                for (int j = 0; j < Np; ++j) for (int i = 0; i < Np; ++i) dpsi(i, j) = Yd(i, j);
+                gemm(-(sig[n] ? 1.0 : 0.0) * h2_6 * wn(n), sn, psi, 1.0, dpsi);
+            }
+            integrands(n, psi, tan ? &dpsi : nullptr);
+            const int di = dump_at(out->n_dump_psi, out->dump_psi_n, n);
+            put_at(out->dump_psi, static_cast<size_t>(Np) * Np, di, psi);
+            if (tan) put_at(out->dump_dpsi, static_cast<size_t>(Np) * Np, di, dpsi);
+            if (tan) dstore(n, dpsi);
+            store(n, std::move(psi));
         }
     }
     auto get_psi = [&](int n) -> const Mat& {
         if (n >= asym_off && n <= N) return psi_asym[n - asym_off];
-        if (n >= 0 && n <= n_main_hi) return psi_keep[n];
-        throw std::runtime_error("get_psi outside kept ranges");
+        throw std::runtime_error("get_psi outside the kept tail");
     };
     auto get_dpsi = [&](int n) -> const Mat& {
         if (n >= asym_off && n <= N) return dpsi_asym[n - asym_off];
-        if (n >= 0 && n <= n_main_hi) return dpsi_keep[n];
-        throw std::runtime_error("get_dpsi outside kept ranges");
+        throw std::runtime_error("get_dpsi outside the kept tail");
     };
     const double t4 = now_s(); ts[4] = t4 - t3;
     // ---- AsymptoticAmplitudes::extract (AsymptoticAmplitudes.cpp:64-359) ----
@@ -1390,110 +1624,7 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
         }
     }
     const double t5 = now_s(); ts[5] = t5 - t4;
-    // ---- DipoleMatrixElement::compute_six (DipoleMatrixElement.cpp:584-965) ----
-    // initial state and bystander orbitals (main.cpp:746-822)
-    const int Nlm_init = S.n_lambda;
-    const int n_chi = S.n_tr;
-    Mat chi_init(Nr, Nlm_init);
-    std::vector<Mat> phi(n_occ, Mat(Nr, Nlm_init));
-    for (int j = 0; j < n_occ; ++j)
-        for (int ir = 0; ir < n_chi; ++ir)
-            for (int lam = 0; lam < Nlm_init; ++lam) phi[j](ir, lam) = S.chi[ir](j, lam);
-    if (S.P.has_init) {
-        // /initial_state/psi_lm read raw (no r factor), main.cpp:781-802
-        const int Nr_take = std::min(n_chi, Nr);
-        for (int ir = 0; ir < Nr_take; ++ir)
-            for (int lam = 0; lam < Nlm_init; ++lam) chi_init(ir, lam) = S.P.init_psi_lm[static_cast<size_t>(lam) * Nr + ir];
-    } else {
-        for (int ir = 0; ir < n_chi; ++ir)
-            for (int lam = 0; lam < Nlm_init; ++lam) chi_init(ir, lam) = S.chi[ir](n_occ - 1, lam);
-    }
-    Mat dchi(Nr, Nlm_init);
-    {
-        std::vector<double> f(Nr), df(Nr);
-        for (int nu = 0; nu < Nlm_init; ++nu) {
-            for (int ir = 0; ir < Nr; ++ir) f[ir] = chi_init(ir, nu);
-            five_point_derivative(f.data(), Nr, h, df.data());
-            for (int ir = 0; ir < Nr; ++ir) dchi(ir, nu) = df[ir];
-        }
-    }
-    std::vector<int> l_nu(Nlm_init), m_nu(Nlm_init);
-    for (int nu = 0; nu < Nlm_init; ++nu) idx_to_lm(nu, l_nu[nu], m_nu[nu]);
-    auto velocity_coef = [](int lm, int ln) -> double {    // DipoleMatrixElement.cpp:90-95
-        if (lm == ln + 1) return -static_cast<double>(ln + 1);
-        if (lm == ln - 1) return static_cast<double>(ln);
-        return 0.0;
-    };
-    const int qs[3] = {+1, -1, 0};
-    std::vector<Mat> Ang(3, Mat(Np, Nlm_init));
-    for (int qi = 0; qi < 3; ++qi)
-        for (int mu = 0; mu < Np; ++mu)
-            for (int nu = 0; nu < Nlm_init; ++nu) {
-                if (std::abs(S.l_mu[mu] - l_nu[nu]) != 1) continue;
-                const double a = std::sqrt(4.0 * M_PI / 3.0) * gaunt_real(S.l_mu[mu], S.m_mu[mu], 1, qs[qi], l_nu[nu], m_nu[nu]);
-                if (a != 0.0) Ang[qi](mu, nu) = a;
-            }
-    const int n_hi = std::min(Nr, n_main_hi + 1), n_lo = 0, n_pts = n_hi - n_lo;
-    if (n_pts < 5) throw std::runtime_error("DipoleMatrixElement: too few radial points");
-    std::vector<Mat> Xi(6, Mat(n_pts, Np));
-    for (int slot = 0; slot < 6; ++slot) {
-        const Mat& Ag = Ang[slot % 3];
-        const bool L = slot < 3;
-        for (int ir = n_lo; ir < n_hi; ++ir) {
-            const double r = h + ir * h, inv_r = (r > 1e-30) ? 1.0 / r : 0.0;
-            for (int mu = 0; mu < Np; ++mu) {
-                double s = 0.0;
-                for (int nu = 0; nu < Nlm_init; ++nu) {
-                    const double a = Ag(mu, nu);
-                    if (a == 0.0) continue;
-                    const double c = chi_init(ir, nu);
-                    if (L) s += a * c;
-                    else s += a * (dchi(ir, nu) + velocity_coef(S.l_mu[mu], l_nu[nu]) * inv_r * c);
-                }
-                Xi[slot](ir - n_lo, mu) = s;
-            }
-        }
-    }
-    // integrands
-    std::vector<Mat> integ(6, Mat(n_pts, Np));           // [slot](row, beta)
-    std::vector<Mat> integ_b(n_occ, Mat(n_pts, Np));     // [alpha](row, beta)
-    std::vector<Mat> dinteg(tan ? 6 : 0, Mat(n_pts, Np)), dinteg_b(tan ? n_occ : 0, Mat(n_pts, Np));
-    for (int ir = n_hi - 1; ir >= n_lo; --ir) {
-        const int row = ir - n_lo;
-        const double r = h + ir * h;
-        const Mat& ps = get_psi(ir);
-        for (int slot = 0; slot < 6; ++slot) {
-            const double rf = (slot < 3) ? r : 1.0;
-            for (int b = 0; b < Np; ++b) {
-                double s = 0.0;
-                for (int mu = 0; mu < Np; ++mu) s += Xi[slot](row, mu) * ps(mu, b);
-                integ[slot](row, b) = rf * s;
-            }
-        }
-        for (int a = 0; a < n_occ; ++a)
-            for (int b = 0; b < Np; ++b) {
-                double s = 0.0;
-                for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * ps(mu, b);
-                integ_b[a](row, b) = s;
-            }
-        if (tan) {
-            const Mat& dps = get_dpsi(ir);
-            for (int slot = 0; slot < 6; ++slot) {
-                const double rf = (slot < 3) ? r : 1.0;
-                for (int b = 0; b < Np; ++b) {
-                    double s = 0.0;
-                    for (int mu = 0; mu < Np; ++mu) s += Xi[slot](row, mu) * dps(mu, b);
-                    dinteg[slot](row, b) = rf * s;
-                }
-            }
-            for (int a = 0; a < n_occ; ++a)
-                for (int b = 0; b < Np; ++b) {
-                    double s = 0.0;
-                    for (int mu = 0; mu < Np; ++mu) s += phi[a](ir, mu) * dps(mu, b);
-                    dinteg_b[a](row, b) = s;
-                }
-        }
-    }
+    // ---- DipoleMatrixElement::compute_six (DipoleMatrixElement.cpp:584-965): Simpson sums ----
     Mat bov(Np, n_occ), dbov(tan ? Np : 0, tan ? n_occ : 0);
     for (int a = 0; a < n_occ; ++a) for (int b = 0; b < Np; ++b) bov(b, a) = simpson(integ_b[a].col(b), n_pts, h);
     if (tan)
@@ -1579,13 +1710,6 @@ void Scan::run(const Solver& S, double k, sehf_oracle_out* out) {
         d[4] = S.n_tr; d[5] = ns; d[6] = n_fs; d[7] = n_fe; d[8] = shA; d[9] = shS; d[10] = n_closed;
         d[11] = unit_err_kme; d[12] = S_unit;
     }
-    for (int i = 0; i < out->n_dump_sinv; ++i) put(out->dump_sinv + static_cast<size_t>(i) * Np * Np, sinv[out->dump_sinv_n[i]]);
-    for (int i = 0; i < out->n_dump_rinv; ++i) put(out->dump_rinv + static_cast<size_t>(i) * Nt * Nt, rinv[out->dump_rinv_n[i]]);
-    for (int i = 0; i < out->n_dump_psi; ++i) put(out->dump_psi + static_cast<size_t>(i) * Np * Np, get_psi(out->dump_psi_n[i]));
-    if (tan && out->dump_drinv)
-        for (int i = 0; i < out->n_dump_rinv; ++i) put(out->dump_drinv + static_cast<size_t>(i) * Nt * Nt, drinv[out->dump_rinv_n[i]]);
-    if (tan && out->dump_dpsi)
-        for (int i = 0; i < out->n_dump_psi; ++i) put(out->dump_dpsi + static_cast<size_t>(i) * Np * Np, get_dpsi(out->dump_psi_n[i]));
     if (out->t_stage) for (int i = 0; i < 8; ++i) out->t_stage[i] = ts[i];
 }
 
@@ -1599,6 +1723,8 @@ extern "C" {
 const char* sehf_oracle_last_error(void) { return orc::g_err.c_str(); }
 
 void sehf_oracle_set_inverse_mode(int mode) { orc::g_inverse_mode = mode; }
+
+void sehf_oracle_set_threads(int threads) { orc::g_threads = std::max(1, threads); }
 
 int sehf_oracle_solve(const sehf_oracle_problem* p, double k, sehf_oracle_out* out) {
     try {

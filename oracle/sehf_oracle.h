@@ -96,11 +96,21 @@ const char* sehf_oracle_last_error(void);
  * bit 0: Gauss-Jordan with partial pivoting in place of partial-pivot LU for
  *        the dense inverses;
  * bit 1: the asymptotic-fit window sums accumulated from the outer end
- *        inwards (the order of a back-propagation sweep).
+ *        inwards (the order of a back-propagation sweep);
+ * bits 2, 3, 4: Gauss-Jordan for the Schur inverses, the forward ratio
+ *        steps or the K-matrix inverses only;
+ * bit 5: the partial-pivot LU in 80-bit extended precision;
+ * bit 6: the W^{-1} application's product sums in reverse order.
  * Each is a second, equally valid rounding order of the reference algorithm,
  * used by tests only to measure its rounding noise floor.  Process-wide; not
  * thread-safe against concurrent solves. */
 void sehf_oracle_set_inverse_mode(int mode);
+
+/* Threads (default 1) for the loops over independent output columns inside
+ * one solve (dense products, the LU update and the LU solves).  Each
+ * column's operations are unchanged, so results are bitwise those of one
+ * thread.  Process-wide. */
+void sehf_oracle_set_threads(int threads);
 
 /* ---- primitives exposed for golden-vector tests ---- */
 double sehf_oracle_w3j(int j1, int j2, int j3, int m1, int m2, int m3);
