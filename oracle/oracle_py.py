@@ -159,6 +159,19 @@ def noise_floor(p, ks, **kw):
         L.sehf_oracle_set_inverse_mode(0)
 
 
+def solve_scan_rounded_inverses(p, ks, **kw):
+    """Oracle run with every dense inverse evaluated in 80-bit extended
+    precision and rounded once to FP64 (correctly rounded inverses; all other
+    operations as in the reference): the referee for outputs whose FP64
+    value the reference's own inverse rounding already moves."""
+    L = lib()
+    L.sehf_oracle_set_inverse_mode(32)
+    try:
+        return solve_scan(p, ks, **kw)
+    finally:
+        L.sehf_oracle_set_inverse_mode(0)
+
+
 def time_steps(p, k, n_steps):
     L = lib()
     ref = ProblemRef(p)

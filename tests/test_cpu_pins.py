@@ -152,6 +152,43 @@ def test_dense_primitives(O, H):
     assert abs(H.sehf_host_min_eig(70, Sf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) - ex) < 1e-12
 
 
+def _exact_inverse(A):
+    """Inverse of the FP64 matrix A in exact rational arithmetic."""
+    from fractions import Fraction
+    n = A.shape[0]
+    M = [[Fraction(float(A[i, j])) for j in range(n)] + [Fraction(int(i == j)) for j in range(n)] for i in range(n)]
+    for k in range(n):
+        p = next(i for i in range(k, n) if M[i][k] != 0)
+        M[k], M[p] = M[p], M[k]
+        piv = M[k][k]
+        M[k] = [v / piv for v in M[k]]
+        for i in range(n):
+            if i != k and M[i][k] != 0:
+                f = M[i][k]
+                M[i] = [a - f * b for a, b in zip(M[i], M[k])]
+    return np.array([[float(M[i][n + j]) for j in range(n)] for i in range(n)])
+
+
+def test_rounded_inverse_mode(O):
+    """Rounding-probe bit 5 (the referee of the device's K-family outputs):
+    the 80-bit LU inverse rounded once to FP64 is within 1e-13 of the exact
+    inverse of an ill-conditioned FP64 matrix (Hilbert, order 7, condition
+    ~5e8; measured 7e-14), where the FP64 LU is off by 6e-10."""
+    n = 7
+    Hm = np.array([[1.0 / (i + j + 1) for j in range(n)] for i in range(n)])
+    X = _exact_inverse(Hm)
+    lu = O.inverse_general(Hm)
+    L = O.lib()
+    L.sehf_oracle_set_inverse_mode(32)
+    try:
+        ext = O.inverse_general(Hm)
+    finally:
+        L.sehf_oracle_set_inverse_mode(0)
+    e_lu, e_ext = rel_err(lu, X), rel_err(ext, X)
+    print(f"FP64 LU {e_lu:.1e}, rounded {e_ext:.1e}")
+    assert e_ext < 1e-13 and e_lu > 100 * e_ext
+
+
 @pytest.fixture(scope="module")
 def small():
     from sehf import synth

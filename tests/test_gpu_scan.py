@@ -5,14 +5,17 @@ d_lm in both gauges, at the target tolerance 1e-9 (max-abs error over
 max-abs reference), plus the integer diagnostics (n_transition, n_start, fit
 window, stability-shift counts, closed-channel count) exactly.
 
-Noise floor.  The energy enters the Schur matrix only as (h^2/6) E ~ 1e-7 on an
-O(1) diagonal, so an epsilon-level rounding difference anywhere in Sinv is a
-relative energy shift of ~eps / ((h^2/6) E) (1e-9 at k = 0.1).  Two correct
-rounding orders of the reference itself (the oracle with LU, and with
-Gauss-Jordan elimination) therefore differ in K by up to ~1e-9 at the lowest
-k.  For the quantities that inherit this (K, A, B, K_fit, b_overlap, d_raw)
-the bound is max(1e-9, 4 x that measured spread at the same k); d_lm keeps the
-flat 1e-9.
+The reduced dipoles d_lm are held to a flat 1e-9 against the FP64 oracle (the
+reference's algorithm as it is).  The energy enters the Schur matrix only as
+(h^2/6) E ~ 1e-7 on an O(1) diagonal, so the rounding of the dense inverses
+moves K, A, B, K_fit, b_overlap and d_raw by ~eps / ((h^2/6) E): at k = 0.1 the
+FP64 oracle's K moves by 2.8e-9 when its inverses are evaluated exactly
+rounded, and on the closed-channel mid-size case by 1.6e-7, while K_fit and
+d_lm stay below 2e-10.  Those outputs are therefore compared with the same
+oracle run with correctly rounded inverses (80-bit LU rounded once to FP64,
+every other operation unchanged), within max(1e-9, the FP64 oracle's own
+distance to it): the device must be at least as accurate as the reference's
+FP64 algorithm, and on the l_cont = 8 configuration within a flat 1e-9.
 """
 import numpy as np
 import pytest
@@ -23,30 +26,31 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-9
 
+KEYS = ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D")
+ORACLE_KEY = {"K": "K_kme"}
 
-FLOOR_KEYS = ("K", "A", "B", "K_fit", "b_overlap", "d_raw")
 
-
-def _compare(p, ks, got, ref_list, tol=TOL, label="", floor_list=None):
+def _compare(p, ks, got, ref_list, tol=TOL, label="", ext_list=None, strict=False):
+    """ref_list: the FP64 oracle.  ext_list: the oracle with correctly rounded
+    inverses; with it, every output but D is checked against ext_list within
+    max(tol, FP64 oracle's distance to ext_list), or tol when strict."""
     worst = {}
     for e, (g, r) in enumerate(zip(got, ref_list)):
-        bound = {}
-        if floor_list is not None:
-            f = floor_list[e]
-            fp = {"K": (f["K_kme"], r["K_kme"]), "A": (f["A"], r["A"]), "B": (f["B"], r["B"]),
-                  "K_fit": (f["K_fit"], r["K_fit"]), "b_overlap": (f["b_overlap"], r["b_overlap"]),
-                  "d_raw": (f["d_raw"], r["d_raw"])}
-            bound = {key: max(tol, 4.0 * rel_err(a, b)) for key, (a, b) in fp.items()}
-        pairs = {"K": (g["K"], r["K_kme"]), "A": (g["A"], r["A"]), "B": (g["B"], r["B"]),
-                 "K_fit": (g["K_fit"], r["K_fit"]), "b_overlap": (g["b_overlap"], r["b_overlap"]),
-                 "d_raw": (g["d_raw"], r["d_raw"]), "D": (g["D"], r["D"])}
-        per = {}
-        for key, (a, b) in pairs.items():
-            per[key] = rel_err(a, b)
-            lim = bound.get(key, tol)
-            worst[key] = max(worst.get(key, 0.0), per[key] / lim * tol)
-        print(f"  {label} k={g['k']:.3f} " + " ".join(f"{k}={v:.1e}" for k, v in per.items())
-              + ("  floor(K)=%.1e" % (bound["K"] / 4.0) if bound else ""))
+        per, lim = {}, {}
+        for key in KEYS:
+            ok = ORACLE_KEY.get(key, key)
+            if key == "D" or ext_list is None:
+                per[key], lim[key] = rel_err(g[key], r[ok]), tol
+            else:
+                x = ext_list[e][ok]
+                per[key] = rel_err(g[key], x)
+                lim[key] = tol if strict else max(tol, rel_err(r[ok], x))
+            worst[key] = max(worst.get(key, 0.0), per[key] / lim[key] * tol)
+        line = " ".join(f"{k}={v:.1e}" for k, v in per.items())
+        if ext_list is not None:
+            line += f"  (FP64 oracle vs rounded-inverse oracle: K={rel_err(r['K_kme'], ext_list[e]['K_kme']):.1e}" \
+                    f", D vs FP64 oracle={rel_err(g['D'], r['D']):.1e})"
+        print(f"  {label} k={g['k']:.3f} " + line)
         for idx in (4, 5, 6, 7, 8, 9, 10):
             assert g["diag"][idx] == r["diag"][idx], f"diag[{idx}] {g['diag'][idx]} != {r['diag'][idx]} at k={g['k']}"
         np.testing.assert_allclose(g["diag"][0], r["diag"][0], rtol=1e-6)
@@ -88,8 +92,8 @@ def test_scan_h2_l8_full(ctx):
     p, ks = synth.config("h2_l8")
     got = scan.solve_scan(p, ks, ctx=ctx)
     ref = O.solve_scan(p, ks)
-    floor = O.noise_floor(p, ks)
-    _compare(p, ks, got, ref, label="h2_l8", floor_list=floor)
+    ext = O.solve_scan_rounded_inverses(p, ks)
+    _compare(p, ks, got, ref, label="h2_l8", ext_list=ext, strict=True)
 
 
 def test_scan_groups_bitwise(ctx):
@@ -162,8 +166,9 @@ def test_autotune_then_run(ctx):
 def test_scan_h2o_l8_midsize(ctx):
     """A mid-size system (l_cont = 8, three occupied orbitals: N_psi = 81,
     N_f = 243, N_tot = 324, 501 points) whose order-324 inverses and
-    back-propagation products run on the 128 x 128 GEMM tiles, against the
-    oracle with its measured rounding floor."""
+    back-propagation products run on the 128 x 128 GEMM tiles; closed channels
+    make its two-point K ill-conditioned (the FP64 oracle's K moves by 1.6e-7
+    under correctly rounded inverses at k = 1.05)."""
     import oracle_py as O
     from sehf import scan, synth
     p = synth.h2o_like(l_cont=8, dr=0.02, r_max=10.0, n_occ=3, seed=1, l_orb_sce=6, allow_closed=True)
@@ -171,8 +176,8 @@ def test_scan_h2o_l8_midsize(ctx):
     ks = np.array([0.55, 1.05])
     got = scan.solve_scan(p, ks, ctx=ctx)
     ref = O.solve_scan(p, ks)
-    floor = O.noise_floor(p, ks)
-    _compare(p, ks, got, ref, label="h2o_l8", floor_list=floor)
+    ext = O.solve_scan_rounded_inverses(p, ks)
+    _compare(p, ks, got, ref, label="h2o_l8", ext_list=ext)
 
 
 @pytest.mark.parametrize("case", ["h2o_l4_pol", "h2_cutoff", "h2o_l8"])

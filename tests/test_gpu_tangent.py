@@ -9,14 +9,14 @@ finite differences and by the analytic-vs-FD Wigner delay
   E, along the polarisation strength alpha, and along a mixed direction, on
   the small H2O-like case with the soft polarisation potential, closed
   channels allowed;
-* the bound is max(1e-8, 4 x the tangent's spread between two rounding
-  orders of the oracle (the reference's: LU, fit sums from the inner end of
-  the window; the other: Gauss-Jordan, fit sums from the outer end))
-  relative (max-abs error over max-abs value).  The tangent inherits the
+* the referee is the oracle's tangent with correctly rounded inverses (80-bit
+  LU rounded once to FP64, every other operation unchanged), and the bound is
+  max(1e-8, the FP64 oracle tangent's own distance to it) relative (max-abs
+  error over max-abs value): the device tangent must be at least as accurate
+  as the reference algorithm's FP64 tangent.  The tangent inherits the
   forward outputs' rounding floor amplified by the conditioning of the ratio
-  recursion; the measured spread on these cases is ~1e-9 .. 5e-7, so a
-  dropped term (all of which are at least O(h^2/6) ~ 7e-5 relative) cannot
-  hide under it;
+  recursion; that distance is 1e-8 .. 2e-6 on these cases, so a dropped term
+  (all of which are at least O(h^2/6) ~ 7e-5 relative) cannot hide under it;
 * dA and dB are measured as one pair (the derivative of the fitted amplitude
   psi ~ A j + B y), normalised by the pair's max-abs: the fit keeps A within
   ~1e-4 of the identity, so max|dA| is ~1e-3 of max|dB| and a per-matrix
@@ -58,16 +58,18 @@ def _errs(got, ref):
 
 
 def _oracle_with_floor(O, p, k, direction):
+    """The oracle's tangent with correctly rounded inverses, and the bound
+    max(1e-8, the FP64 oracle tangent's distance to it) per output."""
     a = O.solve(p, k, tangent=direction)
     L = O.lib()
-    L.sehf_oracle_set_inverse_mode(3)
+    L.sehf_oracle_set_inverse_mode(32)
     try:
-        b = O.solve(p, k, tangent=direction)
+        x = O.solve(p, k, tangent=direction)
     finally:
         L.sehf_oracle_set_inverse_mode(0)
-    spread = _errs(b, a)
-    bound = {key: max(1e-8, 4.0 * spread[key]) for key in KEYS}
-    return a, bound
+    own = _errs(a, x)
+    bound = {key: max(1e-8, own[key]) for key in KEYS}
+    return x, bound
 
 
 def _check(got, ref, bound, label):
