@@ -147,6 +147,48 @@ def cycle_rate(S, p, lay, ks, segs, warm, tangent=None):
     return tf, tr, tb, pts, walls
 
 
+def kernel_table(L, B, calib_mib):
+    """Rates of the path's inner kernels alone at the scan's shapes
+    (sehf_time_gemm, sehf_time_inverse) and, first, one device copy of
+    calib_mib MiB, the known-size reference of HBM byte counters."""
+    import ctypes
+    from sehf._lib import check
+    out = {"batch": B}
+    ms = ctypes.c_double()
+    if calib_mib > 0:
+        check(L.sehf_copy_probe(ctypes.c_size_t(calib_mib << 20), ctypes.byref(ms)), "sehf_copy_probe")
+        out["calibration_copy"] = {"MiB": calib_mib, "ms": ms.value, "GB_per_s": 2 * (calib_mib << 20) / (ms.value * 1e-3) / 1e9}
+    n, npsi = 1566, 961
+    # (name, M, N, K, lda, ldb, ldc, beta) of the l_cont = 30 scan's products
+    shapes = [("Gauss-Jordan update 783x1566x783", 783, n, 783, 783, n, n, 1),
+              ("Gauss-Jordan update 392x1566x391", 392, n, 391, 392, n, n, 1),
+              ("Gauss-Jordan update 196x1566x196", 196, n, 196, 196, n, n, 1),
+              ("Gauss-Jordan update 98x1566x98", 98, n, 98, 98, n, n, 1),
+              ("Gauss-Jordan update 48x1566x48", 48, n, 48, 48, n, n, 1),
+              ("back-propagation Rinv Z 1566x961x1566", n, npsi, n, n, n, n, 0),
+              ("back-propagation Sinv rhs 961^3", npsi, npsi, npsi, npsi, npsi, npsi, 0)]
+    rows = []
+    for name, M, N, K, lda, ldb, ldc, beta in shapes:
+        check(L.sehf_time_gemm(M, N, K, lda, ldb, ldc, B, beta, 3, ctypes.byref(ms)), "sehf_time_gemm")
+        fl = 2.0 * M * N * K * B
+        by = 8.0 * B * (M * K + K * N + (2 if beta else 1) * M * N)
+        rows.append({"shape": name, "us": ms.value * 1e3, "tflops": fl / (ms.value * 1e-3) / 1e12,
+                     "algorithmic_bytes": by})
+        log(f"kernels: {name} batch {B}: {ms.value * 1e3:.1f} us, {rows[-1]['tflops']:.1f} TFLOP/s")
+    out["gemm"] = rows
+    inv = []
+    res = ctypes.c_double()
+    for order in (n, npsi):
+        for b in (8, 16, 32, 48, 64):
+            check(L.sehf_time_inverse(order, b, 3, ctypes.byref(ms), ctypes.byref(res)), "sehf_time_inverse")
+            inv.append({"n": order, "batch": b, "ms": ms.value, "ms_per_matrix": ms.value / b,
+                        "tflops_2n3": 2.0 * order ** 3 * b / (ms.value * 1e-3) / 1e12, "resid": res.value})
+            log(f"kernels: inverse n={order} batch {b}: {ms.value:.2f} ms ({inv[-1]['tflops_2n3']:.1f} TFLOP/s, "
+                f"|AX-I| {res.value:.1e})")
+    out["inverse"] = inv
+    return out
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -206,6 +248,10 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="skip the per-ik, derivative and fixed-stage runs")
     ap.add_argument("--prof-every", type=int, default=97, help="HIP-event sampling of every n-th GEMM/leaf "
                                                                 "launch; 0 = off")
+    ap.add_argument("--kernels", action="store_true",
+                    help="instead of the scan: the inner kernels alone at the scan's shapes (GEMM table, inverse "
+                         "scaling), after a calibration copy of --calib-mib MiB")
+    ap.add_argument("--calib-mib", type=int, default=1024)
     ap.add_argument("--full-scan", action="store_true",
                     help="also run one whole batch through the full grid (minutes) and report its measured rate "
                          "beside the modelled one")
@@ -221,6 +267,11 @@ def main():
     from sehf._lib import check, load
 
     L = load()
+    if args.tiles >= 0:
+        check(L.sehf_set_gemm_tiles(args.tiles), "sehf_set_gemm_tiles")
+    if args.kernels:
+        print(json.dumps({"kernels": kernel_table(L, max(args.batch, 1), args.calib_mib)}), flush=True)
+        return
     t0 = time.time()
     p, ks_all = synth.config(args.config)
     log(f"{args.config}: N_psi={p.n_psi} N_f={p.n_f} N_tot={p.n_tot} Nr={p.Nr} "
@@ -230,8 +281,6 @@ def main():
     t_setup = time.time() - t0
     info = S.info
     log(f"setup (V(r), exchange, dipole tables on device) {t_setup:.1f} s; info {info}")
-    if args.tiles >= 0:
-        check(L.sehf_set_gemm_tiles(args.tiles), "sehf_set_gemm_tiles")
     B, tune = args.batch, None
     nk = len(ks_all)
     if B <= 0:

@@ -19,6 +19,7 @@
 #define SEHF_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -57,6 +58,40 @@ typedef struct sehf_sinv sehf_sinv;
 typedef struct sehf_bp   sehf_bp;
 typedef struct sehf_dme  sehf_dme;
 typedef struct sehf_scan sehf_scan;
+
+/* Accumulated per-phase wall time of a stepper's step() calls, in
+ * nanoseconds, and the number of steps; cleared by the matching
+ * sehf_*_reset_stats.  Field for field the Stats structs of the reference's
+ * steppers: GpuForwardStepper::Stats (src/scatt/GpuPropagate.hpp:133-139),
+ * GpuSinvStepper::Stats (:211-217), GpuBackStepper::Stats (:263-269) and
+ * GpuDipoleEngine::Stats (src/scatt/GpuDipoleEngine.hpp:132-137). */
+typedef struct {
+    uint64_t t_upload_ns;      /* W_inv host -> device                      */
+    uint64_t t_u_combine_ns;   /* U = 12 W_inv - 10 I - Rinv_{n-1}          */
+    uint64_t t_inverse_ns;     /* inverse + symmetrisation                  */
+    uint64_t t_download_ns;    /* Rinv device -> host (+ symmetry check)    */
+    uint64_t n_steps;
+} sehf_frp_stats_t;
+typedef struct {
+    uint64_t t_upload_ns;      /* A, B, Dinv host -> device                 */
+    uint64_t t_schur_ns;       /* S = A - B D^{-1} B^T                      */
+    uint64_t t_inverse_ns;     /* inverse + symmetrisation                  */
+    uint64_t t_download_ns;    /* Sinv device -> host (+ symmetry check)    */
+    uint64_t n_steps;
+} sehf_sinv_stats_t;
+typedef struct {
+    uint64_t t_upload_ns;      /* Rinv, W_inv host -> device                */
+    uint64_t t_gemm_z_ns;      /* Z := Rinv Z                               */
+    uint64_t t_gemm_y_ns;      /* Y := W_inv Z                              */
+    uint64_t t_extract_ns;     /* psi, f rows device -> host                */
+    uint64_t n_steps;
+} sehf_bp_stats_t;
+typedef struct {
+    uint64_t t_upload_ns;
+    uint64_t t_gemm_ns;
+    uint64_t t_download_ns;
+    uint64_t n_steps;
+} sehf_dme_stats_t;
 
 /* Last error message of the calling thread ("" if none). */
 const char* sehf_last_error(void);
@@ -99,6 +134,8 @@ int  sehf_frp_enable_symmetric_inverse(sehf_frp* s, int enable);
 int  sehf_frp_step_noalloc(int N_total, const double* W_inv, const double* R_prev, double* Rinv_out,
                            double* sym_err);
 int  sehf_frp_is_symmetric_inverse_active(const sehf_frp* s);
+int  sehf_frp_stats(const sehf_frp* s, sehf_frp_stats_t* out);   /* stats() */
+int  sehf_frp_reset_stats(sehf_frp* s);                          /* reset_stats() */
 void sehf_frp_destroy(sehf_frp* s);
 
 /* ---- Schur-complement inverse stepper ----------------------------------
@@ -108,6 +145,8 @@ int  sehf_sinv_create(sehf_ctx* ctx, int N_psi, int N_f, sehf_sinv** out);
 int  sehf_sinv_step(sehf_sinv* s, const double* A, const double* B, const double* Dinv,
                     double* Sinv_out, double* sym_err);
 int  sehf_sinv_step_inverse_only(sehf_sinv* s, const double* A, double* Sinv_out, double* sym_err);
+int  sehf_sinv_stats(const sehf_sinv* s, sehf_sinv_stats_t* out);
+int  sehf_sinv_reset_stats(sehf_sinv* s);
 void sehf_sinv_destroy(sehf_sinv* s);
 
 /* ---- back stepper -------------------------------------------------------
@@ -119,6 +158,8 @@ int  sehf_bp_init_Z(sehf_bp* s, const double* Z);
 int  sehf_bp_step(sehf_bp* s, const double* Rinv, const double* W_inv, double* psi_out,
                   double* f_out, int compute_f);
 int  sehf_bp_get_Z(const sehf_bp* s, double* Z_out);
+int  sehf_bp_stats(const sehf_bp* s, sehf_bp_stats_t* out);
+int  sehf_bp_reset_stats(sehf_bp* s);
 void sehf_bp_destroy(sehf_bp* s);
 
 /* ---- dipole engine ------------------------------------------------------
@@ -126,6 +167,8 @@ void sehf_bp_destroy(sehf_bp* s);
  *   result (n_slots x N_psi) = V (n_slots x N_psi) * psi_ir (N_psi x N_psi). */
 int  sehf_dme_create(sehf_ctx* ctx, int N_psi, int n_slots, sehf_dme** out);
 int  sehf_dme_step(sehf_dme* s, const double* V, const double* psi_ir, double* result_out);
+int  sehf_dme_stats(const sehf_dme* s, sehf_dme_stats_t* out);
+int  sehf_dme_reset_stats(sehf_dme* s);
 void sehf_dme_destroy(sehf_dme* s);
 
 /* ---- stage-level batched scan ----------------------------------------
@@ -232,6 +275,25 @@ int  sehf_scan_stats(const sehf_scan* s, double* t);
  * batching it replaces the reference's serial per-ik loop
  * (src/main.cpp:845-877). */
 int  sehf_scan_set_groups(sehf_scan* s, int groups);
+/* How sehf_scan_run sums the dipole integrals d = sum_n rows_n psi_n.
+ * The reference back-propagates Z from the outer boundary over the whole
+ * grid, keeps psi_n (BackPropagator.cpp:553-, the loop n = N-1 ... 0) and
+ * sums rows_n psi_n over the radial points afterwards
+ * (DipoleMatrixElement.cpp:584-720, compute_six).  Since psi_n =
+ * P_n Rinv_n ... Rinv_{N-1} Z_N with P_n = [Sinv_n | -Sinv_n B D^{-1}], the
+ * same sum is
+ *   d = Ad_{N-1} Z_N,   Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n,  Ad_{-1} = 0,
+ * a row block of 6 + n_occ rows carried along the forward recursion.
+ *   SEHF_DIPOLES_FORWARD: the forward accumulation; the only
+ *     back-propagation left is over the fit window at the top of the grid
+ *     (the fit sums, then the fit residual), from the checkpoints of the top
+ *     segments only.
+ *   SEHF_DIPOLES_BACKWARD (default): the back-propagation over the whole grid, with
+ *     checkpoints of every segment and their recompute.
+ * Both give the same outputs to rounding (the sums differ in order). */
+#define SEHF_DIPOLES_FORWARD 0
+#define SEHF_DIPOLES_BACKWARD 1
+int  sehf_scan_set_dipole_path(sehf_scan* s, int path);
 void sehf_scan_destroy(sehf_scan* s);
 
 /* Kernel timing: between begin and end, every `every`-th FP64 GEMM and
@@ -242,6 +304,24 @@ void sehf_scan_destroy(sehf_scan* s);
  * 32 pairs), already subtracted from every sampled duration. */
 int  sehf_prof_begin(int every);
 int  sehf_prof_end(double* out);
+
+/* ---- timing probes of the inner kernels ---------------------------------
+ * Synthetic operands on the default stream, HIP events around the timed
+ * repetitions (after one untimed run); for the benchmark's kernel table and
+ * hardware-counter runs.  No reference counterpart.
+ *   sehf_time_gemm: `batch` column-major products C = A B (+ C when
+ *     beta_one) of M x K by K x N with the given leading dimensions, average
+ *     ms per batched launch sequence (the scan's own GEMM launcher and tile
+ *     policy);
+ *   sehf_time_inverse: the batched Gauss-Jordan inverse of `batch` random
+ *     order-n matrices (diagonal raised by 2), average ms; resid_out (may be
+ *     NULL) = sampled max |A X - I| of the first matrix;
+ *   sehf_copy_probe: one device-to-device copy of `bytes` (reads and writes
+ *     exactly that many bytes: the reference of HBM byte counters). */
+int  sehf_time_gemm(int M, int N, int K, int lda, int ldb, int ldc, int batch, int beta_one, int reps,
+                    double* ms_out);
+int  sehf_time_inverse(int n, int batch, int reps, double* ms_out, double* resid_out);
+int  sehf_copy_probe(size_t bytes, double* ms_out);
 
 /* ---- host-only inputs (no device needed) -------------------------------
  * The energy-independent matrices and scalar tables the scan uploads, for

@@ -6,6 +6,7 @@
 #include "../../include/sehf.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 
@@ -69,6 +70,7 @@ struct sehf_frp {
     DevArray<double> W, U, R, asym;
     GjBuffers gj;
     int sym_request = 0;
+    sehf_frp_stats_t stats{};
 };
 
 struct sehf_sinv {
@@ -76,18 +78,35 @@ struct sehf_sinv {
     int Np, Nf;
     DevArray<double> A, B, Bs, Dinv, Sinv, asym;
     GjBuffers gj;
+    sehf_sinv_stats_t stats{};
 };
 
 struct sehf_bp {
     sehf_ctx* ctx;
     int Nt, Np, Nf;
     DevArray<double> Z, Ztmp, Y, Rinv, Winv;
+    sehf_bp_stats_t stats{};
 };
 
 struct sehf_dme {
     sehf_ctx* ctx;
     int Np, ns;
     DevArray<double> V, psi, res;
+    sehf_dme_stats_t stats{};
+};
+
+// Per-phase wall time of a synchronous step: each mark waits for the stream
+// and adds the time since the previous mark to a counter (nanoseconds).
+struct PhaseClock {
+    hipStream_t st;
+    std::chrono::steady_clock::time_point t0;
+    explicit PhaseClock(hipStream_t s) : st(s), t0(std::chrono::steady_clock::now()) {}
+    void mark(uint64_t& acc) {
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        const auto t = std::chrono::steady_clock::now();
+        acc += static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(t - t0).count());
+        t0 = t;
+    }
 };
 
 static double download_asym(DevArray<double>& a, const double* X, int n, hipStream_t s) {
@@ -167,22 +186,34 @@ int sehf_frp_step(sehf_frp* s, const double* W_inv, double* Rinv_out, double* sy
         hipStream_t st = s->ctx->stream;
         const int N = s->N;
         const size_t nn = static_cast<size_t>(N) * N;
+        PhaseClock pc(st);
         SEHF_HIP_CHECK(hipMemcpyAsync(s->W.p, W_inv, nn * sizeof(double), hipMemcpyHostToDevice, st));
+        pc.mark(s->stats.t_upload_ns);
         // U = 12 W_inv - 10 I - Rinv_{n-1}
         axpbyi(s->U.p, N, 0, s->W.p, N, 0, 12.0, s->R.p, N, 0, -1.0, -10.0, N, N, 1, st);
+        pc.mark(s->stats.t_u_combine_ns);
         fill_int(s->gj.info.p, 1, 0, st);
         gj_inverse(s->U.p, s->R.p, N, N, 0, 1, true, s->gj.work(), st);
         check_gj_info(s->gj, 1, st, "sehf_frp_step");
+        pc.mark(s->stats.t_inverse_ns);
         if (Rinv_out)
             SEHF_HIP_CHECK(hipMemcpyAsync(Rinv_out, s->R.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
         const double a = download_asym(s->asym, s->R.p, N, st);
         if (sym_err) *sym_err = a;
+        pc.mark(s->stats.t_download_ns);
+        ++s->stats.n_steps;
     });
 }
 int sehf_frp_enable_symmetric_inverse(sehf_frp* s, int enable) {
     return guarded([&] { s->sym_request = enable ? 1 : 0; });
 }
 int sehf_frp_is_symmetric_inverse_active(const sehf_frp*) { return 0; }
+int sehf_frp_stats(const sehf_frp* s, sehf_frp_stats_t* out) {
+    return guarded([&] { *out = s->stats; });
+}
+int sehf_frp_reset_stats(sehf_frp* s) {
+    return guarded([&] { s->stats = sehf_frp_stats_t{}; });
+}
 void sehf_frp_destroy(sehf_frp* s) { delete s; }
 
 // ---------------- Sinv stepper ----------------
@@ -197,26 +228,33 @@ int sehf_sinv_create(sehf_ctx* ctx, int N_psi, int N_f, sehf_sinv** out) {
         *out = s;
     });
 }
-static void sinv_common(sehf_sinv* s, double* Sinv_out, double* sym_err) {
+static void sinv_common(sehf_sinv* s, double* Sinv_out, double* sym_err, PhaseClock& pc) {
     hipStream_t st = s->ctx->stream;
     const int Np = s->Np;
     fill_int(s->gj.info.p, 1, 0, st);
     gj_inverse(s->A.p, s->Sinv.p, Np, Np, 0, 1, true, s->gj.work(), st);
     check_gj_info(s->gj, 1, st, "sehf_sinv_step");
+    pc.mark(s->stats.t_inverse_ns);
     if (Sinv_out)
         SEHF_HIP_CHECK(hipMemcpyAsync(Sinv_out, s->Sinv.p, sizeof(double) * Np * Np, hipMemcpyDeviceToHost, st));
     const double a = download_asym(s->asym, s->Sinv.p, Np, st);
     if (sym_err) *sym_err = a;
+    pc.mark(s->stats.t_download_ns);
+    ++s->stats.n_steps;
 }
 int sehf_sinv_step(sehf_sinv* s, const double* A, const double* B, const double* Dinv, double* Sinv_out,
                    double* sym_err) {
     return guarded([&] {
         hipStream_t st = s->ctx->stream;
         const int Np = s->Np, Nf = s->Nf;
+        PhaseClock pc(st);
         SEHF_HIP_CHECK(hipMemcpyAsync(s->A.p, A, sizeof(double) * Np * Np, hipMemcpyHostToDevice, st));
         if (Nf > 0) {
             SEHF_HIP_CHECK(hipMemcpyAsync(s->B.p, B, sizeof(double) * Np * Nf, hipMemcpyHostToDevice, st));
             SEHF_HIP_CHECK(hipMemcpyAsync(s->Dinv.p, Dinv, sizeof(double) * Nf, hipMemcpyHostToDevice, st));
+        }
+        pc.mark(s->stats.t_upload_ns);
+        if (Nf > 0) {
             col_scale(s->Bs.p, Np, s->B.p, Np, s->Dinv.p, Np, Nf, st);
             Gemm g;   // S = A - (B D^{-1}) B^T
             g.M = Np; g.N = Np; g.K = Nf;
@@ -224,15 +262,24 @@ int sehf_sinv_step(sehf_sinv* s, const double* A, const double* B, const double*
             g.C = s->A.p; g.ldc = Np; g.alpha = -1.0; g.beta = 1.0;
             gemm(g, st);
         }
-        sinv_common(s, Sinv_out, sym_err);
+        pc.mark(s->stats.t_schur_ns);
+        sinv_common(s, Sinv_out, sym_err, pc);
     });
 }
 int sehf_sinv_step_inverse_only(sehf_sinv* s, const double* A, double* Sinv_out, double* sym_err) {
     return guarded([&] {
         hipStream_t st = s->ctx->stream;
+        PhaseClock pc(st);
         SEHF_HIP_CHECK(hipMemcpyAsync(s->A.p, A, sizeof(double) * s->Np * s->Np, hipMemcpyHostToDevice, st));
-        sinv_common(s, Sinv_out, sym_err);
+        pc.mark(s->stats.t_upload_ns);
+        sinv_common(s, Sinv_out, sym_err, pc);
     });
+}
+int sehf_sinv_stats(const sehf_sinv* s, sehf_sinv_stats_t* out) {
+    return guarded([&] { *out = s->stats; });
+}
+int sehf_sinv_reset_stats(sehf_sinv* s) {
+    return guarded([&] { s->stats = sehf_sinv_stats_t{}; });
 }
 void sehf_sinv_destroy(sehf_sinv* s) { delete s; }
 
@@ -259,23 +306,34 @@ int sehf_bp_step(sehf_bp* s, const double* Rinv, const double* W_inv, double* ps
         hipStream_t st = s->ctx->stream;
         const int Nt = s->Nt, Np = s->Np, Nf = s->Nf;
         const size_t tt = static_cast<size_t>(Nt) * Nt;
+        PhaseClock pc(st);
         SEHF_HIP_CHECK(hipMemcpyAsync(s->Rinv.p, Rinv, tt * sizeof(double), hipMemcpyHostToDevice, st));
         SEHF_HIP_CHECK(hipMemcpyAsync(s->Winv.p, W_inv, tt * sizeof(double), hipMemcpyHostToDevice, st));
+        pc.mark(s->stats.t_upload_ns);
         Gemm g;
         g.M = Nt; g.N = Np; g.K = Nt;
         g.A = s->Rinv.p; g.lda = Nt; g.B = s->Z.p; g.ldb = Nt; g.C = s->Ztmp.p; g.ldc = Nt;
         gemm(g, st);
         std::swap(s->Z.p, s->Ztmp.p);
+        pc.mark(s->stats.t_gemm_z_ns);
         g.A = s->Winv.p; g.B = s->Z.p; g.C = s->Y.p;
         gemm(g, st);
+        pc.mark(s->stats.t_gemm_y_ns);
         if (psi_out)
             SEHF_HIP_CHECK(hipMemcpy2DAsync(psi_out, sizeof(double) * Np, s->Y.p, sizeof(double) * Nt,
                                             sizeof(double) * Np, Np, hipMemcpyDeviceToHost, st));
         if (compute_f && f_out && Nf > 0)
             SEHF_HIP_CHECK(hipMemcpy2DAsync(f_out, sizeof(double) * Nf, s->Y.p + Np, sizeof(double) * Nt,
                                             sizeof(double) * Nf, Np, hipMemcpyDeviceToHost, st));
-        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        pc.mark(s->stats.t_extract_ns);
+        ++s->stats.n_steps;
     });
+}
+int sehf_bp_stats(const sehf_bp* s, sehf_bp_stats_t* out) {
+    return guarded([&] { *out = s->stats; });
+}
+int sehf_bp_reset_stats(sehf_bp* s) {
+    return guarded([&] { s->stats = sehf_bp_stats_t{}; });
 }
 int sehf_bp_get_Z(const sehf_bp* s, double* Z_out) {
     return guarded([&] {
@@ -300,15 +358,25 @@ int sehf_dme_step(sehf_dme* s, const double* V, const double* psi_ir, double* re
     return guarded([&] {
         hipStream_t st = s->ctx->stream;
         const int Np = s->Np, ns = s->ns;
+        PhaseClock pc(st);
         SEHF_HIP_CHECK(hipMemcpyAsync(s->V.p, V, sizeof(double) * ns * Np, hipMemcpyHostToDevice, st));
         SEHF_HIP_CHECK(hipMemcpyAsync(s->psi.p, psi_ir, sizeof(double) * Np * Np, hipMemcpyHostToDevice, st));
+        pc.mark(s->stats.t_upload_ns);
         Gemm g;
         g.M = ns; g.N = Np; g.K = Np;
         g.A = s->V.p; g.lda = ns; g.B = s->psi.p; g.ldb = Np; g.C = s->res.p; g.ldc = ns;
         gemm(g, st);
+        pc.mark(s->stats.t_gemm_ns);
         SEHF_HIP_CHECK(hipMemcpyAsync(result_out, s->res.p, sizeof(double) * ns * Np, hipMemcpyDeviceToHost, st));
-        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        pc.mark(s->stats.t_download_ns);
+        ++s->stats.n_steps;
     });
+}
+int sehf_dme_stats(const sehf_dme* s, sehf_dme_stats_t* out) {
+    return guarded([&] { *out = s->stats; });
+}
+int sehf_dme_reset_stats(sehf_dme* s) {
+    return guarded([&] { s->stats = sehf_dme_stats_t{}; });
 }
 void sehf_dme_destroy(sehf_dme* s) { delete s; }
 

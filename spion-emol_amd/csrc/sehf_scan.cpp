@@ -50,6 +50,7 @@ struct Batch {
     DevArray<double> ckpt, segR, segS, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
     DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
+    DevArray<double> Ad, Ad2, Tq;   // forward-accumulated dipole rows (ns x Nt, ping-pong) and rows_n Sinv_n
     GjBuffers gj;             // inversions on the main stream
     GjBuffers gj2;            // Schur inversions on the side stream
     // tangent along (E, alpha), allocated by the first tangent run
@@ -57,6 +58,7 @@ struct Batch {
     DevArray<double> dRp, dRc, dckpt, dsegR, Gt, dZ, dZt, P1, dPsi, dacc, dYJ, dYY, dAm, dBm, dKd, dP, Q1, Q2;
     DevArray<double> vre, vim, dDRre, dDRim, dore, doim, dDre, dDim;
     DevArray<double> wF, wP, wN, danl, dtab, djy, dJc, dYc, dcoef;
+    DevArray<double> dAd, dAd2;
 };
 
 // One energy group: its own stream and batch buffers.  Groups run
@@ -108,6 +110,7 @@ struct sehf_scan {
     std::vector<std::unique_ptr<Group>> groups;
     int n_groups = 1;
     int ctx_device = 0;
+    int dipole_path = SEHF_DIPOLES_BACKWARD;   // how the dipole integrals are summed (sehf_scan_set_dipole_path)
     std::mutex host_mu;   // lazily computed host tables (stability eigenvalues)
     size_t dev_bytes_shared = 0;
     double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
@@ -342,9 +345,10 @@ void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_ou
     if (meS < thr) { dS_out = thr - meS + Setup::shift_margin; ++shS; }
 }
 
-void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, bool tan) {
+// n_ck: segment start matrices (checkpoints) kept per energy.
+void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_ck, int n_fit, bool tan) {
     Batch& b = g.bt;
-    if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_seg, 1)) * B * s->Nt * s->Nt &&
+    if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_ck, 1)) * B * s->Nt * s->Nt &&
         b.Jc.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np && (!tan || b.has_tan))
         return;
     const size_t Np = s->Np, Nf = s->Nf, Nt = s->Nt, ns = s->nslot;
@@ -357,7 +361,7 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, boo
     b.T.alloc(pp * B); b.Psi.alloc(pp * B); b.acc.alloc(ns * Np * B);
     b.YJ.alloc(pp * B); b.YY.alloc(pp * B); b.Am.alloc(pp * B); b.Bm.alloc(pp * B); b.Kd.alloc(pp * B);
     b.P.alloc(pp * B); b.Vpsi.alloc(pp * B); b.G.alloc(pf * B); b.Bf.alloc(pp * B);
-    b.ckpt.alloc(tt * B * std::max(n_seg, 1));
+    b.ckpt.alloc(tt * B * std::max(n_ck, 1));
     b.segR.alloc(tt * B * L); b.segS.alloc(pp * B * L);
     b.W2.alloc(4 * pp * B); b.W2i.alloc(4 * pp * B);
     b.Rff.alloc(ff * B); b.Rffi.alloc(ff * B); b.T1.alloc(pf * B);
@@ -372,11 +376,12 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, boo
     b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
     b.gj2.ensure(static_cast<int>(Np), B);
     b.SinvR.alloc(2 * pp * B);
+    b.Ad.alloc(ns * Nt * B); b.Ad2.alloc(ns * Nt * B); b.Tq.alloc(ns * Np * B);
     if (tan) {
         const size_t no = s->S->n_occ, nx = std::max<size_t>(6, no);
         b.has_tan = true;
         b.dRp.alloc(tt * B); b.dRc.alloc(tt * B);
-        b.dckpt.alloc(tt * B * std::max(n_seg, 1)); b.dsegR.alloc(tt * B * L);
+        b.dckpt.alloc(tt * B * std::max(n_ck, 1)); b.dsegR.alloc(tt * B * L);
         b.Gt.alloc(Np * Nt * B); b.dZ.alloc(Nt * Np * B); b.dZt.alloc(Nt * Np * B);
         b.P1.alloc(pp * B); b.dPsi.alloc(pp * B); b.dacc.alloc(ns * Np * B);
         b.dYJ.alloc(pp * B); b.dYY.alloc(pp * B); b.dAm.alloc(pp * B); b.dBm.alloc(pp * B); b.dKd.alloc(pp * B);
@@ -389,6 +394,7 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_seg, int n_fit, boo
         b.dtab.alloc(4 * Np * B); b.djy.alloc(2 * Np * B);
         b.dJc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.dYc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
         b.dcoef.alloc(4 * Np * B);
+        b.dAd.alloc(ns * Nt * B); b.dAd2.alloc(ns * Nt * B);
     }
     g.ensure_seg_events(L);
 }
@@ -411,6 +417,16 @@ void fit_window(int Nr, int& n_fs, int& n_fe) {
     n_fs = std::max(n_fs, asym_off);
     n_fe = std::min(n_fe, asym_off + 300 - 1);
     if (n_fe - n_fs + 1 < 10) throw std::runtime_error("asymptotic fit window is too small");
+}
+
+// First segment whose start matrix is kept: every segment when the dipoles
+// are back-propagated over the whole grid, else only the segments from the
+// one holding the fit window's first point (the only back-propagation left).
+int ckpt_lo(const sehf_scan* s, int L) {
+    if (s->dipole_path == SEHF_DIPOLES_BACKWARD) return 0;
+    int n_fs = 0, n_fe = 0;
+    fit_window(s->Nr, n_fs, n_fe);
+    return n_fs / L;
 }
 
 // One batch of energies through the whole path.  With `dir` (per energy the
@@ -758,34 +774,128 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // the analytic start matrix (a ratio matrix of the right size and
     // conditioning; the cycle's results are not kept).
     const int n_first = seg >= 0 ? std::max(ns0, seg * L) : ns0;
+    // Checkpoint sg (the start matrix Rinv_{sg L - 1} of segment sg) for
+    // sg >= ck_lo; a segment cycle keeps its one start matrix in slot 0.
+    const int ck_lo = ckpt_lo(s, L);
+    auto ckp = [&](double* base, int sg) {
+        const int i = seg >= 0 ? 0 : sg - ck_lo;
+        return base + static_cast<size_t>(i) * nb * tt;
+    };
+    // Below n_start the ratio matrices are the analytic diagonal; a whole scan
+    // with forward-accumulated dipoles still needs those points' Sinv_n for
+    // the dipole rows, so its forward loop starts at n = 1.
+    const bool adj = s->dipole_path == SEHF_DIPOLES_FORWARD;
+    const int n_lo = (adj && seg < 0 && probe == 0) ? std::min(1, n_first) : n_first;
     auto sinv_ahead = [&](int n) {
         const int slot = n & 1;
-        if (n - 2 >= n_first) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
+        if (n - 2 >= n_lo) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
         sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
     };
     const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
                                  : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
-    if (n_first <= n_last) sinv_ahead(n_first);
-    for (int n = n_first; n <= n_last; ++n) {
+    // Forward accumulation of the dipole integrals.  psi_n = P_n Z_n with
+    // P_n = [Sinv_n | X_n] (X_n = -Sinv_n B D^{-1}, zero from n_tr on) and
+    // Z_n = Rinv_n ... Rinv_{N-1} Z_N, so sum_{n<N} rows_n psi_n = Ad_{N-1} Z_N
+    // with Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n, Ad_{-1} = 0 (Rinv_0 = 0 makes
+    // Ad_0 = 0).  Its tangent, with dRinv_n = Rinv_n H_n Rinv_n for n >= n_start
+    // (H_n = dRinv_{n-1} + wF_n G G^T, what frp_tangent leaves in its H) and
+    // d(rows_n P_n) = wP_n (rows_n Sinv_n) P_n:
+    //   dAd_n = (dAd_{n-1} + wP_n T_n P_n + Ad_n H_n) Rinv_n,  T_n = rows_n Sinv_n;
+    // below n_start, where Rinv_n is the analytic diagonal,
+    //   dAd_n = (dAd_{n-1} + wP_n T_n P_n) Rinv_n + (Ad_{n-1} + rows_n P_n) dRinv_n.
+    // Ad, dAd are (6 + n_occ) x N_tot per energy (ld ns).
+    const int64_t sAd = static_cast<int64_t>(ns) * Nt;
+    auto gm = [&](double* C, int64_t ldc, int64_t sC, const double* A, int64_t lda, int64_t sA, const double* Bm,
+                  int64_t ldb, int64_t sB, int M, int Nn, int K, double alpha, double beta, const double* alpha_b) {
+        Gemm q;
+        q.M = M; q.N = Nn; q.K = K;
+        q.A = A; q.lda = lda; q.sA = sA; q.B = Bm; q.ldb = ldb; q.sB = sB;
+        q.C = C; q.ldc = ldc; q.sC = sC; q.alpha = alpha; q.beta = beta; q.alpha_b = alpha_b; q.batch = nb;
+        gemm(q, st);
+    };
+    // Ad_{n-1} += rows_n P_n (and T_n = rows_n Sinv_n kept for the tangent)
+    auto adj_rows = [&](int n, const double* Sv) {
+        if (n >= s->n_pts) return;
+        const double* rows = s->Drows.p + static_cast<size_t>(n) * ns * Np;
+        if (tan) {
+            gm(b.Tq.p, ns, static_cast<int64_t>(ns) * Np, rows, ns, 0, Sv, Np, pp, ns, Np, Np, 1.0, 0.0, nullptr);
+            axpy_b(b.Ad.p, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, nullptr, 1.0, ns, Np, nb, st);
+        } else {
+            gm(b.Ad.p, ns, sAd, rows, ns, 0, Sv, Np, pp, ns, Np, Np, 1.0, 1.0, nullptr);
+        }
+        if (n < s->n_tr && Nf > 0)
+            gm(b.Ad.p + static_cast<size_t>(ns) * Np, ns, sAd, rows, ns, 0, Xd, Np, sXd, ns, Nf, Np, 1.0, 1.0, nullptr);
+    };
+    // dAd_{n-1} += wP_n T_n P_n
+    auto adj_drows = [&](int n, const double* Sv) {
+        if (n >= s->n_pts) return;
+        const double* wPn = b.wP.p + static_cast<size_t>(n) * nb;
+        gm(b.dAd.p, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, Sv, Np, pp, ns, Np, Np, 1.0, 1.0, wPn);
+        if (n < s->n_tr && Nf > 0)
+            gm(b.dAd.p + static_cast<size_t>(ns) * Np, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, Xd, Np, sXd,
+               ns, Nf, Np, 1.0, 1.0, wPn);
+    };
+    auto adj_times_R = [&](double* out, const double* in, const double* Rn, double beta) {
+        gm(out, ns, sAd, in, ns, sAd, Rn, Nt, tt, ns, Nt, Nt, 1.0, beta, nullptr);
+    };
+    if (adj) {
+        fill(b.Ad.p, static_cast<size_t>(sAd) * nb, 0.0, st);
+        if (tan) fill(b.dAd.p, static_cast<size_t>(sAd) * nb, 0.0, st);
+    }
+    if (n_lo <= n_last) sinv_ahead(n_lo);
+    for (int n = n_lo; n <= n_last; ++n) {
         if (n + 1 <= n_last) sinv_ahead(n + 1);
         const int slot = n & 1;
         SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
         const double* Sslot = b.SinvR.p + static_cast<size_t>(slot) * pp * nb;
+        if (n < ns0) {
+            // analytic region: Rinv_n = diag(anl_n) built in Rc (Rp holds the
+            // start matrix Rinv_{n_start-1} and is not touched)
+            if (n < s->n_tr && Nf > 0)
+                gm(Xd, Np, sXd, Sslot, Np, pp, s->BD.p + static_cast<size_t>(n) * pf, Np, 0, Np, Nf, Np, -1.0, 0.0,
+                   nullptr);
+            adj_rows(n, Sslot);
+            diag_matrix(b.Rc.p, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+            adj_times_R(b.Ad2.p, b.Ad.p, b.Rc.p, 0.0);
+            if (tan) {
+                diag_matrix(b.dRc.p, b.danl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+                adj_drows(n, Sslot);
+                adj_times_R(b.dAd2.p, b.dAd.p, b.Rc.p, 0.0);
+                adj_times_R(b.dAd2.p, b.Ad.p, b.dRc.p, 1.0);
+                std::swap(b.dAd.p, b.dAd2.p);
+            }
+            std::swap(b.Ad.p, b.Ad2.p);
+            SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
+            continue;
+        }
         frp_step(n, Sslot, b.Rp.p, b.Rc.p);
-        if (tan) frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);
+        if (adj) {
+            adj_rows(n, Sslot);
+            adj_times_R(b.Ad2.p, b.Ad.p, b.Rc.p, 0.0);
+        }
+        if (tan) {
+            frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);   // dRp now holds H_n
+            if (adj) {
+                adj_drows(n, Sslot);
+                gm(b.dAd.p, ns, sAd, b.Ad2.p, ns, sAd, b.dRp.p, Nt, tt, ns, Nt, Nt, 1.0, 1.0, nullptr);   // += Ad_n H_n
+                adj_times_R(b.dAd2.p, b.dAd.p, b.Rc.p, 0.0);
+                std::swap(b.dAd.p, b.dAd2.p);
+            }
+        }
+        if (adj) std::swap(b.Ad.p, b.Ad2.p);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
         std::swap(b.Rp.p, b.Rc.p);
         if (tan) std::swap(b.dRp.p, b.dRc.p);
         progress("forward sweep n =", n, Nr - 2);
         if ((n + 1) % L == 0) {
             const int sidx = (n + 1) / L;
-            if (sidx < n_seg) {
-                SEHF_HIP_CHECK(hipMemcpyAsync(b.ckpt.p + static_cast<size_t>(sidx) * nb * tt, b.Rp.p, nb * tt * sizeof(double),
+            if (sidx < n_seg && sidx >= ck_lo) {
+                SEHF_HIP_CHECK(hipMemcpyAsync(ckp(b.ckpt.p, sidx), b.Rp.p, nb * tt * sizeof(double),
                                               hipMemcpyDeviceToDevice, st));
                 if (tan)
-                    SEHF_HIP_CHECK(hipMemcpyAsync(b.dckpt.p + static_cast<size_t>(sidx) * nb * tt, b.dRp.p,
-                                                  nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    SEHF_HIP_CHECK(hipMemcpyAsync(ckp(b.dckpt.p, sidx), b.dRp.p, nb * tt * sizeof(double),
+                                                  hipMemcpyDeviceToDevice, st));
             }
         }
     }
@@ -940,6 +1050,16 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (N < s->n_pts && seg < 0) dip_accum(N, b.dP.p, b.dacc.p);
         if (N >= n_fs && N <= n_fe && seg < 0) fit_tangent(N, b.P.p, b.dP.p);
     }
+    if (adj && seg < 0) {
+        // the forward-accumulated rows applied to the outer boundary:
+        // sum_{n<N} rows_n psi_n = Ad_{N-1} Z_N, its tangent dAd Z_N + Ad dZ_N
+        const int64_t sZN = static_cast<int64_t>(Nt) * Np, sAc = static_cast<int64_t>(ns) * Np;
+        gm(b.acc.p, ns, sAc, b.Ad.p, ns, sAd, b.Z.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
+        if (tan) {
+            gm(b.dacc.p, ns, sAc, b.dAd.p, ns, sAd, b.Z.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
+            gm(b.dacc.p, ns, sAc, b.Ad.p, ns, sAd, b.dZ.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
+        }
+    }
     fill(b.red.p, 2 * nb, 0.0, st);
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw3 = wall();
@@ -971,28 +1091,45 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         gemm(g, st);
     };
     // ---------------- back-propagation by segments ----------------
-    // sweep(main = false): the fit pre-pass, segments from the top down to the
-    // one holding n_fs, psi only, stopping at n_fs.  sweep(main = true): the
-    // whole grid with the dipole sums, the fit residual and the tangent.
-    // A segment cycle runs the main pass over its one segment (sg_hi = sg_lo)
+    // Three kinds of pass, each over segments sg_hi down to sg_lo with every
+    // segment's ratio matrices recomputed from its checkpoint:
+    //   FitSums: from the top down to the fit window's first point n_fs, psi
+    //     (and, with forward-accumulated dipoles, the tangent) into the
+    //     window sums;
+    //   Resid:   the same points again once A, B are known, the fit residual;
+    //   Full:    the whole grid with the dipole sums, the fit residual and the
+    //     tangent (back-propagated dipoles; a segment cycle's pass).
+    // A segment cycle runs a Full pass over its one segment (sg_hi = sg_lo)
     // and, with t_rec set, records when the segment's recompute has finished.
+    enum class Pass { FitSums, Resid, Full };
     bool seg_pending = false;   // a previous segment's backward products may still read segS
     double t_rec = 0.0;
-    auto sweep = [&](bool main, int sg_hi, int sg_lo, bool time_rec) {
+    auto sweep = [&](Pass kind, int sg_hi, int sg_lo, bool time_rec) {
+        const bool main = kind == Pass::Full;
         for (int sg = sg_hi; sg >= sg_lo; --sg) {
             const int n0 = sg * L, n1 = std::min(n0 + L, N);
-            const double* prev = (sg == 0) ? nullptr : b.ckpt.p + static_cast<size_t>(sg) * nb * tt;
-            const bool tg = main && tan;
-            const double* dprev0 = (sg == 0 || !tg) ? nullptr : b.dckpt.p + static_cast<size_t>(sg) * nb * tt;
+            const double* prev = (sg == 0) ? nullptr : ckp(b.ckpt.p, sg);
+            const bool tg = tan && (main || (kind == Pass::FitSums && adj));
+            const double* dprev0 = (sg == 0 || !tg) ? nullptr : ckp(b.dckpt.p, sg);
             const double* dprev = dprev0;
             // the segment's Schur inverses on the side stream, after the
-            // previous segment's backward products are done with segS
+            // previous segment's backward products are done with segS.  They
+            // are enqueued a few points ahead of the recompute that consumes
+            // them, so the host starts the main stream's ratio steps after a
+            // handful of inverses instead of the whole segment's (each is a
+            // few hundred launches).
             if (seg_pending) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
+            constexpr int kAhead = 4;
+            int n_enq = n0;
+            auto enqueue_sinv_to = [&](int n_hi) {
+                for (; n_enq < std::min(n_hi, n1); ++n_enq) {
+                    sinv_step(n_enq, b.segS.p + static_cast<size_t>(n_enq - n0) * nb * pp);
+                    SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n_enq - n0], st2));
+                }
+            };
+            enqueue_sinv_to(n0 + kAhead);
             for (int n = n0; n < n1; ++n) {
-                sinv_step(n, b.segS.p + static_cast<size_t>(n - n0) * nb * pp);
-                SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n - n0], st2));
-            }
-            for (int n = n0; n < n1; ++n) {
+                enqueue_sinv_to(n + 1 + kAhead);
                 double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
                 double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
                 double* dRn = tg ? b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt : nullptr;
@@ -1018,7 +1155,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 SEHF_HIP_CHECK(hipStreamSynchronize(st));
                 t_rec = wall();
             }
-            progress(main ? "back-propagation segment" : "fit pre-pass segment", sg, n_seg);
+            progress(main ? "back-propagation segment" : kind == Pass::FitSums ? "fit pre-pass segment"
+                                                                                : "fit residual segment",
+                     sg, n_seg);
             const int n_stop = main ? n0 : std::max(n0, n_fs);
             for (int n = n1 - 1; n >= n_stop; --n) {
                 const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
@@ -1027,13 +1166,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 std::swap(b.Z.p, b.Zt.p);
                 winv_psi(n, b.Z.p, Sn, b.Psi.p);                                     // psi_n
                 const bool in_win = n >= n_fs && n <= n_fe;
-                if (!main) {
-                    if (in_win) fit_sums(n, b.Psi.p);
-                    continue;
+                if (main && n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
+                if (in_win) {
+                    if (kind == Pass::FitSums) fit_sums(n, b.Psi.p);
+                    else fit_resid_at(n, b.Psi.p);
                 }
-                if (n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
-                if (in_win) fit_resid_at(n, b.Psi.p);
-                if (!tan) continue;
+                if (!tg) continue;
                 // dZ_n = dRinv_n Z_{n+1} + Rinv_n dZ_{n+1}.  For n >= n_start
                 // dRinv_n = Rinv_n H_n Rinv_n, H_n = dRinv_{n-1} + wF_n G G^T, and
                 // G^T Z_n = psi_n, so
@@ -1066,7 +1204,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 // dpsi_n = (W_n^{-1} dZ_n)_psi + wP_n Sinv_n psi_n
                 winv_psi(n, b.dZ.p, Sn, b.dPsi.p);
                 axpy_b(b.dPsi.p, Np, pp, b.P1.p, Np, pp, b.wP.p + static_cast<size_t>(n) * nb, 1.0, Np, Np, nb, st);
-                if (n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
+                if (main && n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
                 if (in_win) fit_tangent(n, b.Psi.p, b.dPsi.p);
             }
             SEHF_HIP_CHECK(hipEventRecord(g.ev_bwd, st));   // segS free for the next segment's Sinv
@@ -1081,10 +1219,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         // those of the scan's main pass over this segment.
         const int64_t sZs = static_cast<int64_t>(Nt) * Np;
         if (seg > 0) {
-            double* c0 = b.ckpt.p + static_cast<size_t>(seg) * nb * tt;
+            double* c0 = ckp(b.ckpt.p, seg);
             if (ns0 >= 2) diag_matrix(c0, b.anl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
             else fill(c0, tt * nb, 0.0, st);
-            if (tan) fill(b.dckpt.p + static_cast<size_t>(seg) * nb * tt, tt * nb, 0.0, st);
+            if (tan) fill(ckp(b.dckpt.p, seg), tt * nb, 0.0, st);
         }
         fill(b.Z.p, sZs * nb, 0.0, st);
         axpbyi(b.Z.p, Nt, sZs, b.Z.p, Nt, sZs, 0.0, nullptr, 0, 0, 0.0, 1.0, Np, Np, nb, st);
@@ -1094,7 +1232,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         fill(b.Bm.p, pp * nb, 0.0, st);
         SEHF_HIP_CHECK(hipStreamSynchronize(st));
         const double tb0 = wall();
-        sweep(true, seg, seg, true);
+        sweep(Pass::Full, seg, seg, true);
         SEHF_HIP_CHECK(hipStreamSynchronize(st2));
         SEHF_HIP_CHECK(hipStreamSynchronize(st));
         check_gj_info(b.gj, nb, st, "segment cycle");
@@ -1103,12 +1241,13 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (seg_t) { seg_t[2] = t_rec - tb0; seg_t[3] = tb1 - t_rec; }
         return;
     }
-    sweep(false, n_seg - 1, n_fs / L, false);
-    // A, B of the fit from the completed window sums; the main pass then
-    // starts again from the outer boundary
+    sweep(Pass::FitSums, n_seg - 1, n_fs / L, false);
+    // A, B of the fit from the completed window sums; the next pass starts
+    // again from the outer boundary
     fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
     zmap(b.P.p, b.Z.p);
-    sweep(true, n_seg - 1, 0, false);
+    if (adj) sweep(Pass::Resid, n_seg - 1, n_fs / L, false);
+    else sweep(Pass::Full, n_seg - 1, 0, false);
     check_gj_info(b.gj, nb, st, "scan back-propagation");
     check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
@@ -1310,7 +1449,7 @@ void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batc
         s->groups.emplace_back(new Group());
         s->groups.back()->init();
     }
-    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, dir != nullptr);
+    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L), n_fit, dir != nullptr);
     if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
     // batch i runs on group i % G; each group is driven by its own host thread
     std::vector<std::string> errs(G);
@@ -1361,7 +1500,7 @@ void scan_segment(sehf_scan* s, const double* k, const double* dir, int nk, int 
         s->groups.back()->init();
     }
     SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
-    alloc_batch(s, *s->groups[0], nk, L, n_seg, 200, dir != nullptr);
+    alloc_batch(s, *s->groups[0], nk, L, n_seg - ckpt_lo(s, L), 200, dir != nullptr);
     double tt[4] = {0, 0, 0, 0};
     run_batch(s, *s->groups[0], k, dir, nk, L, nullptr, nullptr, 0, 0, nullptr, seg, tt);
     if (t) std::copy(tt, tt + 4, t);
@@ -1411,7 +1550,8 @@ int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, i
                     s->groups.emplace_back(new Group());
                     s->groups.back()->init();
                 }
-                for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg, n_fit, tangent != 0);
+                for (int gi = 0; gi < G; ++gi)
+                    alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L), n_fit, tangent != 0);
             } catch (const std::exception&) {
                 (void)hipGetLastError();
                 return -1.0;
@@ -1517,6 +1657,15 @@ int sehf_scan_set_groups(sehf_scan* s, int groups) {
         if (groups < 1 || groups > 4)
             throw std::runtime_error("sehf_scan_set_groups: groups must be in [1, 4] (one stream each)");
         s->n_groups = groups;
+    });
+}
+
+int sehf_scan_set_dipole_path(sehf_scan* s, int path) {
+    return guarded([&] {
+        if (path != SEHF_DIPOLES_FORWARD && path != SEHF_DIPOLES_BACKWARD)
+            throw std::runtime_error("sehf_scan_set_dipole_path: path must be SEHF_DIPOLES_FORWARD (0) or "
+                                     "SEHF_DIPOLES_BACKWARD (1)");
+        s->dipole_path = path;
     });
 }
 

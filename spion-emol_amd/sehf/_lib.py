@@ -41,6 +41,14 @@ SIGNATURES = [
     ("sehf_dme_create", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("sehf_dme_step", ctypes.c_int, [_vp, _dp, _dp, _dp]),
     ("sehf_dme_destroy", None, [_vp]),
+    ("sehf_frp_stats", ctypes.c_int, [_vp, _vp]),
+    ("sehf_frp_reset_stats", ctypes.c_int, [_vp]),
+    ("sehf_sinv_stats", ctypes.c_int, [_vp, _vp]),
+    ("sehf_sinv_reset_stats", ctypes.c_int, [_vp]),
+    ("sehf_bp_stats", ctypes.c_int, [_vp, _vp]),
+    ("sehf_bp_reset_stats", ctypes.c_int, [_vp]),
+    ("sehf_dme_stats", ctypes.c_int, [_vp, _vp]),
+    ("sehf_dme_reset_stats", ctypes.c_int, [_vp]),
     ("sehf_scan_create", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     ("sehf_scan_info", ctypes.c_int, [_vp, _ip]),
     ("sehf_scan_run", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
@@ -52,8 +60,12 @@ SIGNATURES = [
     ("sehf_scan_destroy", None, [_vp]),
     ("sehf_scan_stats", ctypes.c_int, [_vp, _dp]),
     ("sehf_scan_set_groups", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("sehf_scan_set_dipole_path", ctypes.c_int, [_vp, ctypes.c_int]),
     ("sehf_prof_begin", ctypes.c_int, [ctypes.c_int]),
     ("sehf_prof_end", ctypes.c_int, [_dp]),
+    ("sehf_time_gemm", ctypes.c_int, [ctypes.c_int] * 9 + [_dp]),
+    ("sehf_time_inverse", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    ("sehf_copy_probe", ctypes.c_int, [ctypes.c_size_t, _dp]),
     ("sehf_host_potential", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("sehf_host_exchange_B", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("sehf_host_sizes", ctypes.c_int, [_vp, _ip]),

@@ -134,6 +134,16 @@ class Scan:
             res.append(r)
         return res
 
+    DIPOLES_FORWARD = 0    # SEHF_DIPOLES_FORWARD
+    DIPOLES_BACKWARD = 1   # SEHF_DIPOLES_BACKWARD
+
+    def set_dipole_path(self, path: int):
+        """How the dipole integrals are summed: DIPOLES_FORWARD carries the adjoint row block Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n
+        along the forward recursion and back-propagates only over the fit
+        window; DIPOLES_BACKWARD (default) back-propagates over the whole
+        grid (sehf_scan_set_dipole_path)."""
+        check(load().sehf_scan_set_dipole_path(self._h, int(path)), "sehf_scan_set_dipole_path")
+
     def layout(self, seg_len: int = 0) -> dict:
         """Segment layout of the radial grid: L points per segment, the number
         of segments, the fit window's first point and the first segment of

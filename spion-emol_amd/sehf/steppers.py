@@ -24,6 +24,31 @@ def _f(a, shape=None):
     return a
 
 
+# Field names of the steppers' per-phase timing counters (nanoseconds), in
+# the order of the sehf_*_stats_t structs of include/sehf.h.
+_STATS_FIELDS = {
+    "frp": ("t_upload_ns", "t_u_combine_ns", "t_inverse_ns", "t_download_ns", "n_steps"),
+    "sinv": ("t_upload_ns", "t_schur_ns", "t_inverse_ns", "t_download_ns", "n_steps"),
+    "bp": ("t_upload_ns", "t_gemm_z_ns", "t_gemm_y_ns", "t_extract_ns", "n_steps"),
+    "dme": ("t_upload_ns", "t_gemm_ns", "t_download_ns", "n_steps"),
+}
+
+
+class _Stats:
+    """stats() / reset_stats() of the reference's steppers (GpuPropagate.hpp
+    Stats structs): accumulated per-phase nanoseconds and the step count."""
+    _kind = ""
+
+    def stats(self) -> dict:
+        names = _STATS_FIELDS[self._kind]
+        buf = (ctypes.c_uint64 * len(names))()
+        check(getattr(load(), f"sehf_{self._kind}_stats")(self._h, ctypes.cast(buf, _vp)), "stats")
+        return dict(zip(names, (int(v) for v in buf)))
+
+    def reset_stats(self):
+        check(getattr(load(), f"sehf_{self._kind}_reset_stats")(self._h), "reset_stats")
+
+
 class GpuContext:
     def __init__(self, prefer_gpu: bool = True):
         L = load()
@@ -48,7 +73,9 @@ class GpuContext:
             self._h = None
 
 
-class GpuForwardStepper:
+class GpuForwardStepper(_Stats):
+    _kind = "frp"
+
     def __init__(self, ctx: GpuContext, N_total: int):
         self.ctx, self.N = ctx, N_total
         h = _vp()
@@ -79,7 +106,9 @@ class GpuForwardStepper:
             self._h = None
 
 
-class GpuSinvStepper:
+class GpuSinvStepper(_Stats):
+    _kind = "sinv"
+
     def __init__(self, ctx: GpuContext, N_psi: int, N_f: int):
         self.ctx, self.Np, self.Nf = ctx, N_psi, N_f
         h = _vp()
@@ -110,7 +139,9 @@ class GpuSinvStepper:
             self._h = None
 
 
-class GpuBackStepper:
+class GpuBackStepper(_Stats):
+    _kind = "bp"
+
     def __init__(self, ctx: GpuContext, N_total: int, N_psi: int, N_f: int):
         self.ctx, self.Nt, self.Np, self.Nf = ctx, N_total, N_psi, N_f
         h = _vp()
@@ -142,7 +173,9 @@ class GpuBackStepper:
             self._h = None
 
 
-class GpuDipoleEngine:
+class GpuDipoleEngine(_Stats):
+    _kind = "dme"
+
     def __init__(self, ctx: GpuContext, N_psi: int, n_slots: int):
         self.ctx, self.Np, self.ns = ctx, N_psi, n_slots
         h = _vp()

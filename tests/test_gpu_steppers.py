@@ -126,3 +126,33 @@ def test_nonsymmetric_inverse_via_sinv_inverse_only(ctx):
     got, _ = st.step_inverse_only(A)
     ref = np.linalg.inv(A)
     assert rel_err(got, 0.5 * (ref + ref.T)) < 1e-10
+
+
+def test_stepper_stats(ctx):
+    """stats() / reset_stats() (GpuPropagate.hpp Stats, GpuDipoleEngine.hpp
+    Stats): per-phase nanoseconds accumulate over steps, n_steps counts the
+    steps, and reset clears every field."""
+    from sehf.steppers import GpuBackStepper, GpuDipoleEngine, GpuForwardStepper, GpuSinvStepper
+    rng = np.random.default_rng(21)
+    N, Np, Nf = 96, 40, 56
+    fr = GpuForwardStepper(ctx, N)
+    fr.init_R_prev_inv(make_random_spd(N, 1))
+    for k in range(3):
+        fr.step(make_random_spd(N, 2 + k))
+    si = GpuSinvStepper(ctx, Np, Nf)
+    si.step(make_random_spd(Np, 5) + 5 * np.eye(Np), 0.1 * rng.standard_normal((Np, Nf)), np.full(Nf, 0.5))
+    si.step_inverse_only(make_random_spd(Np, 6))
+    bp = GpuBackStepper(ctx, N, Np, Nf)
+    bp.init_Z(rng.standard_normal((N, Np)))
+    bp.step(make_random_spd(N, 7), make_random_spd(N, 8), compute_f=True)
+    de = GpuDipoleEngine(ctx, Np, 7)
+    de.step(rng.standard_normal((7, Np)), rng.standard_normal((Np, Np)))
+    for obj, n, phases in ((fr, 3, ("t_upload_ns", "t_u_combine_ns", "t_inverse_ns", "t_download_ns")),
+                           (si, 2, ("t_upload_ns", "t_schur_ns", "t_inverse_ns", "t_download_ns")),
+                           (bp, 1, ("t_upload_ns", "t_gemm_z_ns", "t_gemm_y_ns", "t_extract_ns")),
+                           (de, 1, ("t_upload_ns", "t_gemm_ns", "t_download_ns"))):
+        s = obj.stats()
+        print(type(obj).__name__, s)
+        assert s["n_steps"] == n and all(s[p] > 0 for p in phases), s
+        obj.reset_stats()
+        assert all(v == 0 for v in obj.stats().values())
