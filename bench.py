@@ -5,23 +5,28 @@ Workload: the l_cont = 30 H2O-like scan (961 channels, N_total = 1566, 6001
 radial points at dr = 0.01, k in [0.05, 2.0], closed channels allowed), with
 the energy-independent inputs resident in HBM.
 
-A scan takes each batch of B energies through three sweeps of the radial
-grid: the forward ratio recursion (Schur inverses one step ahead on a side
-stream), and, segment by segment from the top, the recompute of the
-segment's ratio matrices from its checkpoint and the back-propagation with
-the fused dipole and fit work; plus per-batch work (host tables, K matrix,
-fit pre-pass over the top segments, fit solve, dipole normalisation).
+A scan takes each batch of B energies through the forward ratio recursion
+over the whole radial grid (Schur inverses one step ahead on a side stream),
+carrying the dipole integrals forward as an adjoint row block
+(sehf_scan_set_dipole_path, SEHF_DIPOLES_FORWARD); then, over the top
+segments only, two passes of recompute (the segment's ratio matrices from its
+checkpoint) and back-propagation down to the fit window: the fit sums, then
+the fit residual; plus per-batch work (host tables, K matrix, fit solve,
+dipole normalisation).  With --dipole-path backward the dipoles are instead
+back-propagated over the whole grid: recompute and back-propagation of every
+segment after one fit-sum pass over the top ones.
 
 A timed step is one segment cycle at full size (sehf_scan_run_segment): the
-B energies run segment s of all three sweeps -- L = ceil(sqrt(Nr-1)) = 78
-radial points of forward steps, of recompute and of back-propagation.  The
+B energies run segment s of the forward recursion, its recompute and its
+back-propagation -- L = ceil(sqrt(Nr-1)) = 78 radial points of each.  The
 timed steps visit segments spread evenly over the grid (inside and beyond
 the exchange cut-off), so their mean cost per radial point is the grid's.
 `value` (energy points per second) is B over the modelled time of one whole
-batch: the measured per-point cost of the cycles times the grid's Nr-1
-points, plus the fit pre-pass (recompute and back-propagation of the top
-segments, from the same measured per-point costs), plus the per-batch fixed
-stages timed on a full short-grid scan of the same channel sizes.
+batch: the measured per-point forward cost times the grid's Nr-1 points, plus
+the top-segment passes from the measured per-point recompute and
+back-propagation costs, plus the per-batch fixed stages timed on a full
+short-grid scan of the same channel sizes.  --full-scan also runs one whole
+batch through the full grid and reports its measured rate beside the model.
 
 Prints one JSON line.  `roofline` is for the dominant kernel, the FP64 MFMA
 GEMM k_dgemm (every Gauss-Jordan block update and every back-propagation
@@ -58,30 +63,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
+def model_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ, n_pts, n_back):
     """Algorithmic FP64 flop of one energy point (the path's flop model):
     Sinv: sum_n 2Np^3 + [n<n_tr] 2Np^2 Nf
     FRP : sum_{n>=n_start} 2Nt^3 + [n<n_tr] 2 Np Nt (2Nf + Np)
-    BP  : sum_n 2Nt^2 Np + 2Np^3 + [n<n_tr] 4 Np^2 Nf
-    dipole: n_tr (6 + n_occ) 2 Np^2 ;  K, fit, M: 10 Nt^3."""
+    BP  : over the n_back back-propagated points 2Nt^2 Np + 2Np^3 + [n<n_tr] 4 Np^2 Nf
+    dipole rows: sum_{n<n_pts} (6 + n_occ) 2 Np (Np + Nf), and with the
+        forward-accumulated dipoles (n_back < Nr) the row block times Rinv_n,
+        sum_n (6 + n_occ) 2 Nt^2;  K, fit, M: 10 Nt^3.
+    Recomputed ratio matrices and the second (residual) pass over the fit
+    window are not counted."""
     ex = min(n_tr, Nr)
+    ns = 6 + n_occ
     sinv = Nr * 2 * Np ** 3 + ex * 2 * Np ** 2 * Nf
     frp = (Nr - n_start) * 2 * Nt ** 3 + max(0, ex - n_start) * 2 * Np * Nt * (2 * Nf + Np)
-    bp = Nr * (2 * Nt ** 2 * Np + 2 * Np ** 3) + ex * 4 * Np ** 2 * Nf
-    dip = ex * (6 + n_occ) * 2 * Np ** 2
+    bp = n_back * (2 * Nt ** 2 * Np + 2 * Np ** 3) + min(ex, n_back) * 4 * Np ** 2 * Nf
+    dip = n_pts * ns * 2 * Np * (Np + Nf)
+    if n_back < Nr:
+        dip += Nr * ns * 2 * Nt ** 2
     return float(sinv + frp + bp + dip + 10 * Nt ** 3)
-
-
-def tangent_flops_per_energy(Np, Nf, Nt, Nr, n_tr, n_start, n_occ):
-    """Algorithmic FP64 flop the forward-mode derivative adds per energy (the
-    forward model's counting: no credit for the back-propagation recompute):
-    ratio steps: sum_{n>=n_start} G G^T 2Nt^2 Np + two products 4Nt^3
-    back-propagation: sum_n dRinv Z and Rinv(..) 4Nt^2 Np, Sinv psi and the
-        W^{-1} apply of dZ 4Np^3, [n<n_tr] 4Np^2 Nf ; dipole: n_tr (6+n_occ) 2Np^2."""
-    ex = min(n_tr, Nr)
-    fwd = (Nr - n_start) * (2 * Nt ** 2 * Np + 4 * Nt ** 3)
-    bp = Nr * (4 * Nt ** 2 * Np + 4 * Np ** 3) + ex * 4 * Np ** 2 * Nf
-    return float(fwd + bp + ex * (6 + n_occ) * 2 * Np ** 2)
 
 
 def spread_segments(n_seg, count, offset=0.5):
@@ -94,20 +94,28 @@ def seg_points(lay, N, s):
     return min((s + 1) * L, N) - s * L
 
 
-def batch_seconds(lay, N, pts, t_fwd, t_rec, t_back, t_fixed):
-    """Modelled seconds of one whole batch from segment-cycle timings:
-    per-point costs of the three sweeps over the N = Nr-1 points, the fit
-    pre-pass (recompute of the top segments, back-propagation from the top
-    down to the fit window's first point) and the fixed per-batch stages."""
+def batch_seconds(lay, N, pts, t_fwd, t_rec, t_back, t_fixed, forward_dipoles=True):
+    """Modelled seconds of one whole batch from segment-cycle timings.
+    Forward-accumulated dipoles: the forward sweep over the N = Nr-1 points
+    and two passes over the top segments (recompute from the checkpoint of
+    the segment holding the fit window's first point, back-propagation down
+    to that point): fit sums, then fit residual.  Back-propagated dipoles:
+    all three sweeps over the N points plus one such pass (the fit sums).
+    Plus the fixed per-batch stages.  The back-propagation of the top passes
+    is priced at the cycle's (which also forms the dipole integrands)."""
     L, n_seg, nfs, s_pre = lay["L"], lay["n_seg"], lay["n_fit_lo"], lay["prepass_seg_lo"]
     P = float(np.sum(pts))
     f, r, b = np.sum(t_fwd) / P, np.sum(t_rec) / P, np.sum(t_back) / P
-    pre_rec = N - s_pre * L          # points recomputed by the pre-pass
-    pre_back = N - nfs               # points back-propagated by the pre-pass
-    sweeps = N * (f + r + b)
-    prepass = pre_rec * r + pre_back * b
-    return sweeps + prepass + t_fixed, {"per_point_s": {"forward": f, "recompute": r, "back": b},
-                                        "sweeps_s": sweeps, "prepass_s": prepass, "fixed_s": t_fixed}
+    pre_rec = N - s_pre * L          # points recomputed by one top pass
+    pre_back = N - nfs               # points back-propagated by one top pass
+    if forward_dipoles:
+        sweeps = N * f
+        top = 2 * (pre_rec * r + pre_back * b)
+    else:
+        sweeps = N * (f + r + b)
+        top = pre_rec * r + pre_back * b
+    return sweeps + top + t_fixed, {"per_point_s": {"forward": f, "recompute": r, "back": b},
+                                    "sweeps_s": sweeps, "top_passes_s": top, "fixed_s": t_fixed}
 
 
 def fixed_stage_seconds(p, ks, B):
@@ -147,10 +155,12 @@ def cycle_rate(S, p, lay, ks, segs, warm, tangent=None):
     return tf, tr, tb, pts, walls
 
 
-def kernel_table(L, B, calib_mib):
+def kernel_table(L, B, calib_mib, parts="all"):
     """Rates of the path's inner kernels alone at the scan's shapes
-    (sehf_time_gemm, sehf_time_inverse) and, first, one device copy of
-    calib_mib MiB, the known-size reference of HBM byte counters."""
+    (sehf_time_gemm, sehf_time_inverse, sehf_time_spd_inverse) and, first,
+    one device copy of calib_mib MiB, the known-size reference of HBM byte
+    counters.  parts: "all", "gemm" (the product shapes only, so that counter
+    passes see no other launch of the same grid) or "inverse"."""
     import ctypes
     from sehf._lib import check
     out = {"batch": B}
@@ -168,7 +178,7 @@ def kernel_table(L, B, calib_mib):
               ("back-propagation Rinv Z 1566x961x1566", n, npsi, n, n, n, n, 0),
               ("back-propagation Sinv rhs 961^3", npsi, npsi, npsi, npsi, npsi, npsi, 0)]
     rows = []
-    for name, M, N, K, lda, ldb, ldc, beta in shapes:
+    for name, M, N, K, lda, ldb, ldc, beta in (shapes if parts in ("all", "gemm") else []):
         check(L.sehf_time_gemm(M, N, K, lda, ldb, ldc, B, beta, 3, ctypes.byref(ms)), "sehf_time_gemm")
         fl = 2.0 * M * N * K * B
         by = 8.0 * B * (M * K + K * N + (2 if beta else 1) * M * N)
@@ -176,6 +186,8 @@ def kernel_table(L, B, calib_mib):
                      "algorithmic_bytes": by})
         log(f"kernels: {name} batch {B}: {ms.value * 1e3:.1f} us, {rows[-1]['tflops']:.1f} TFLOP/s")
     out["gemm"] = rows
+    if parts == "gemm":
+        return out
     inv = []
     res = ctypes.c_double()
     for order in (n, npsi):
@@ -186,6 +198,14 @@ def kernel_table(L, B, calib_mib):
             log(f"kernels: inverse n={order} batch {b}: {ms.value:.2f} ms ({inv[-1]['tflops_2n3']:.1f} TFLOP/s, "
                 f"|AX-I| {res.value:.1e})")
     out["inverse"] = inv
+    spd = []
+    for b in (16, 48, 64):
+        check(L.sehf_time_spd_inverse(npsi, b, 3, ctypes.byref(ms), ctypes.byref(res)), "sehf_time_spd_inverse")
+        spd.append({"n": npsi, "batch": b, "ms": ms.value, "ms_per_matrix": ms.value / b,
+                    "tflops_4n3_3": 4.0 / 3.0 * npsi ** 3 * b / (ms.value * 1e-3) / 1e12, "resid": res.value})
+        log(f"kernels: SPD inverse n={npsi} batch {b}: {ms.value:.2f} ms ({spd[-1]['tflops_4n3_3']:.1f} TFLOP/s of "
+            f"4/3 n^3, |AX-I| {res.value:.1e})")
+    out["spd_inverse"] = spd
     return out
 
 
@@ -248,10 +268,13 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="skip the per-ik, derivative and fixed-stage runs")
     ap.add_argument("--prof-every", type=int, default=97, help="HIP-event sampling of every n-th GEMM/leaf "
                                                                 "launch; 0 = off")
-    ap.add_argument("--kernels", action="store_true",
+    ap.add_argument("--kernels", nargs="?", const="all", default=None, choices=("all", "gemm", "inverse"),
                     help="instead of the scan: the inner kernels alone at the scan's shapes (GEMM table, inverse "
-                         "scaling), after a calibration copy of --calib-mib MiB")
+                         "scaling), after a calibration copy of --calib-mib MiB; 'gemm' or 'inverse' for one part")
     ap.add_argument("--calib-mib", type=int, default=1024)
+    ap.add_argument("--dipole-path", choices=("forward", "backward"), default="forward",
+                    help="forward: dipoles accumulated along the forward recursion (default); backward: "
+                         "back-propagated over the whole grid")
     ap.add_argument("--full-scan", action="store_true",
                     help="also run one whole batch through the full grid (minutes) and report its measured rate "
                          "beside the modelled one")
@@ -270,14 +293,17 @@ def main():
     if args.tiles >= 0:
         check(L.sehf_set_gemm_tiles(args.tiles), "sehf_set_gemm_tiles")
     if args.kernels:
-        print(json.dumps({"kernels": kernel_table(L, max(args.batch, 1), args.calib_mib)}), flush=True)
+        print(json.dumps({"kernels": kernel_table(L, max(args.batch, 1), args.calib_mib, args.kernels)}), flush=True)
         return
     t0 = time.time()
     p, ks_all = synth.config(args.config)
     log(f"{args.config}: N_psi={p.n_psi} N_f={p.n_f} N_tot={p.n_tot} Nr={p.Nr} "
         f"(inputs generated in {time.time() - t0:.1f} s)")
+    fwd_dip = args.dipole_path == "forward"
+    dpath = sc.Scan.DIPOLES_FORWARD if fwd_dip else sc.Scan.DIPOLES_BACKWARD
     t0 = time.time()
     S = sc.Scan(p)
+    S.set_dipole_path(dpath)
     t_setup = time.time() - t0
     info = S.info
     log(f"setup (V(r), exchange, dipole tables on device) {t_setup:.1f} s; info {info}")
@@ -331,7 +357,7 @@ def main():
         k1 = ks[B // 2: B // 2 + 1]
         few = spread_segments(lay["n_seg"], 4)
         f1, r1, b1, p1, _ = cycle_rate(S, p, lay, k1, few, warm=[few[1]])
-        t1, _ = batch_seconds(lay, N, p1, f1, r1, b1, 0.0)
+        t1, _ = batch_seconds(lay, N, p1, f1, r1, b1, 0.0, fwd_dip)
         side["per_ik"] = {"unit": "energy points/s", "segments": few, "modelled_energy_seconds": t1,
                           "value_without_fixed_stages": 1.0 / t1}
         log(f"per-ik: {t1:.1f} s per energy (without the fixed stages)")
@@ -342,7 +368,8 @@ def main():
         t_full = time.time() - t0
         if not all(np.all(np.isfinite(r["D"])) for r in res):
             raise SystemExit("non-finite dipole in the full scan")
-        full = {"seconds": t_full, "value": B / t_full, "unit": "energy points/s"}
+        full = {"seconds": t_full, "value": B / t_full, "unit": "energy points/s",
+                "stage_seconds": S.stats()}
         log(f"full scan of the batch: {t_full:.1f} s ({B / t_full:.4f} energy points/s)")
     S.close()   # the side runs build their own device tables
 
@@ -350,28 +377,30 @@ def main():
     if not args.no_side:
         t_fixed, fixed_info = fixed_stage_seconds(p, ks, B)
         log(f"fixed per-batch stages {t_fixed:.2f} s ({fixed_info})")
-    t_batch, model = batch_seconds(lay, N, pts, tf, tr, tb, t_fixed)
+    t_batch, model = batch_seconds(lay, N, pts, tf, tr, tb, t_fixed, fwd_dip)
     value = B / t_batch
     log(f"modelled batch {t_batch:.1f} s -> {value:.4f} energy points/s")
     if "per_ik" in side:
-        tb0, _ = batch_seconds(lay, N, pts, tf, tr, tb, 0.0)
+        tb0, _ = batch_seconds(lay, N, pts, tf, tr, tb, 0.0, fwd_dip)
         side["per_ik"]["batched_over_per_ik"] = (B / tb0) * side["per_ik"]["modelled_energy_seconds"]
 
     mflop = model_flops_per_energy(info["n_psi"], info["n_f"], info["n_tot"], info["Nr"],
-                                   info["n_transition"], info["n_start"], p.n_occ)
+                                   info["n_transition"], info["n_start"], p.n_occ, info["n_pts"],
+                                   (N - lay["n_fit_lo"] + 1) if fwd_dip else info["Nr"])
     if not args.no_side:
         # the forward-mode derivative on the l_cont = 20 polarisation
         # configuration: the same segment cycles with and without the tangent
         pc, kc = synth.config("h2o_l20_pol")
         S = sc.Scan(pc)
+        S.set_dipole_path(dpath)
         lay2 = S.layout()
         kb = kc[(np.arange(B) * 37) % len(kc)]
         few2 = spread_segments(lay2["n_seg"], 4)
         ff, rf, bf, pf_, _ = cycle_rate(S, pc, lay2, kb, few2, warm=[few2[1]])
         ft, rt, bt, pt_, _ = cycle_rate(S, pc, lay2, kb, few2, warm=[few2[1]], tangent=(1.0, 0.0))
         S.close()
-        tb_f, _ = batch_seconds(lay2, pc.Nr - 1, pf_, ff, rf, bf, 0.0)
-        tb_t, _ = batch_seconds(lay2, pc.Nr - 1, pt_, ft, rt, bt, 0.0)
+        tb_f, _ = batch_seconds(lay2, pc.Nr - 1, pf_, ff, rf, bf, 0.0, fwd_dip)
+        tb_t, _ = batch_seconds(lay2, pc.Nr - 1, pt_, ft, rt, bt, 0.0, fwd_dip)
         side["tangent"] = {"workload": f"h2o_l20_pol (l_cont={pc.l_cont}, N_tot={pc.n_tot}, Nr={pc.Nr}), batch {B}, "
                                        f"segments {few2}, without the fixed stages",
                            "forward_energies_per_s": B / tb_f, "with_tangent_energies_per_s": B / tb_t,
@@ -381,19 +410,24 @@ def main():
     workload = (f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), Nr={p.Nr}, dr={p.dr}, "
                 f"k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
                 + ("closed channels allowed" if p.allow_closed else "all channels open")
-                + f"; batch of {B} energies.  A step is one full-size segment cycle: the batch runs {lay['L']} "
-                  f"radial points of forward ratio steps, their recompute and their back-propagation at segment s "
-                  f"(timed segments spread over the {lay['n_seg']} segments of the grid).  value = {B} / T_batch, "
-                  f"T_batch = (Nr-1) x the measured per-point cost of the three sweeps + the fit pre-pass (same "
-                  f"per-point costs) + the per-batch fixed stages (timed on a short-grid scan); a 256-energy scan "
-                  f"takes 256 / value seconds")
+                + f"; batch of {B} energies; dipoles "
+                + ("accumulated along the forward recursion" if fwd_dip else "back-propagated over the whole grid")
+                + f".  A step is one full-size segment cycle: the batch runs {lay['L']} radial points of forward "
+                  f"ratio steps, their recompute and their back-propagation at segment s (timed segments spread "
+                  f"over the {lay['n_seg']} segments of the grid).  value = {B} / T_batch, T_batch = (Nr-1) x the "
+                  f"measured per-point cost of the "
+                + ("forward sweep + two recompute/back-propagation passes over the top segments (fit sums, fit "
+                   "residual)" if fwd_dip else "three sweeps + one recompute/back-propagation pass over the top "
+                                               "segments (fit sums)")
+                + " at the same per-point costs + the per-batch fixed stages (timed on a short-grid scan); a "
+                  "256-energy scan takes 256 / value seconds")
     line = {
         "metric": METRIC,
         "value": value, "unit": "energy points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / max(args.steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "batch_energies": B, "segment_points": lay["L"], "n_segments": lay["n_seg"],
-                   "timed_segments": segs, "tiles": args.tiles},
+                   "timed_segments": segs, "tiles": args.tiles, "dipole_path": args.dipole_path},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS,
                      "traffic": TOP_UPDATE_TRAFFIC.get(B, (None,))[0] if args.config == "h2o_l30" else None,

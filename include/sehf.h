@@ -284,13 +284,18 @@ int  sehf_scan_set_groups(sehf_scan* s, int groups);
  * same sum is
  *   d = Ad_{N-1} Z_N,   Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n,  Ad_{-1} = 0,
  * a row block of 6 + n_occ rows carried along the forward recursion.
- *   SEHF_DIPOLES_FORWARD: the forward accumulation; the only
+ *   SEHF_DIPOLES_FORWARD (default): the forward accumulation; the only
  *     back-propagation left is over the fit window at the top of the grid
  *     (the fit sums, then the fit residual), from the checkpoints of the top
  *     segments only.
- *   SEHF_DIPOLES_BACKWARD (default): the back-propagation over the whole grid, with
+ *   SEHF_DIPOLES_BACKWARD: the back-propagation over the whole grid, with
  *     checkpoints of every segment and their recompute.
- * Both give the same outputs to rounding (the sums differ in order). */
+ * Both give the same outputs to rounding (the sums differ in order).  The
+ * setting applies to sehf_scan_run; sehf_scan_run_tangent always
+ * back-propagates: the tangent of the forward sums carries a ratio matrix's
+ * near-pole growth twice (|dRinv_n| ~ |Rinv_n|^2) where the back-propagated
+ * dZ carries it once, and measures up to 4x less accurate.  A new handle
+ * starts with $SEHF_DIPOLE_PATH (0 or 1) when it is set. */
 #define SEHF_DIPOLES_FORWARD 0
 #define SEHF_DIPOLES_BACKWARD 1
 int  sehf_scan_set_dipole_path(sehf_scan* s, int path);
@@ -316,11 +321,16 @@ int  sehf_prof_end(double* out);
  *   sehf_time_inverse: the batched Gauss-Jordan inverse of `batch` random
  *     order-n matrices (diagonal raised by 2), average ms; resid_out (may be
  *     NULL) = sampled max |A X - I| of the first matrix;
+ *   sehf_time_spd_inverse: the batched inverse of `batch` symmetric positive
+ *     definite order-n matrices (I + a small symmetric perturbation, the
+ *     Schur matrices' conditioning) by block elimination without pivoting,
+ *     the scan's Schur inverse; resid_out as above, of the last matrix;
  *   sehf_copy_probe: one device-to-device copy of `bytes` (reads and writes
  *     exactly that many bytes: the reference of HBM byte counters). */
 int  sehf_time_gemm(int M, int N, int K, int lda, int ldb, int ldc, int batch, int beta_one, int reps,
                     double* ms_out);
 int  sehf_time_inverse(int n, int batch, int reps, double* ms_out, double* resid_out);
+int  sehf_time_spd_inverse(int n, int batch, int reps, double* ms_out, double* resid_out);
 int  sehf_copy_probe(size_t bytes, double* ms_out);
 
 /* ---- host-only inputs (no device needed) -------------------------------

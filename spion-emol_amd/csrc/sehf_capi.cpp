@@ -29,10 +29,15 @@ void check_gj_info(const GjBuffers& gb, int batch, hipStream_t s, const char* wh
         fill_int(gb.info.p, static_cast<size_t>(batch), 0, s);
         SEHF_HIP_CHECK(hipStreamSynchronize(s));
     }
-    for (int b = 0; b < batch; ++b)
-        if (info[b] != 0)
+    for (int b = 0; b < batch; ++b) {
+        if (info[b] > 0)
             throw std::runtime_error(std::string(what) + ": singular matrix (zero pivot at index " +
                                      std::to_string(info[b] - 1) + ", batch entry " + std::to_string(b) + ")");
+        if (info[b] < 0)
+            throw std::runtime_error(std::string(what) + ": matrix not positive definite (non-positive pivot at "
+                                     "index " + std::to_string(-info[b] - 1) + ", batch entry " + std::to_string(b) +
+                                     ")");
+    }
 }
 
 template <class F>

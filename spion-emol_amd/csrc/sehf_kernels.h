@@ -63,6 +63,14 @@ size_t gj_nom_elems(int n);    // ints per matrix: leaf nominees
 size_t gj_stg_elems();         // doubles per matrix: leaf staging
 size_t gj_stgi_elems();        // ints per matrix: leaf move list
 
+// In-place batched inverse of symmetric positive definite matrices by
+// recursive 2 x 2 block elimination without pivoting (4/3 n^3 flop of GEMMs,
+// 64 x 64 diagonal leaves in LDS), symmetrised as 1/2 (X + X^T) when
+// `symmetrize`.  Uses w.tbuf (at least (n/2 + 1)^2 doubles per matrix) and
+// w.info: a non-positive leaf pivot at index k sets info = -(1 + k) if still 0.
+void spd_inverse(double* mat, int n, int64_t ld, int64_t stride, int batch, bool symmetrize, const GjWork& w,
+                 hipStream_t s);
+
 // Y[b] = a * X[b] + c * Z[b] + d * I   (n x m column-major blocks; Z may be null).
 void axpbyi(double* Y, int64_t ldy, int64_t sY, const double* X, int64_t ldx, int64_t sX, double a,
             const double* Z, int64_t ldz, int64_t sZ, double c, double d, int n, int m, int batch,

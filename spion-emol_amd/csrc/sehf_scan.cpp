@@ -50,7 +50,7 @@ struct Batch {
     DevArray<double> ckpt, segR, segS, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
     DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
-    DevArray<double> Ad, Ad2, Tq;   // forward-accumulated dipole rows (ns x Nt, ping-pong) and rows_n Sinv_n
+    DevArray<double> Ad, Ad2;   // forward-accumulated dipole rows (ns x Nt per energy, ping-pong)
     GjBuffers gj;             // inversions on the main stream
     GjBuffers gj2;            // Schur inversions on the side stream
     // tangent along (E, alpha), allocated by the first tangent run
@@ -58,7 +58,6 @@ struct Batch {
     DevArray<double> dRp, dRc, dckpt, dsegR, Gt, dZ, dZt, P1, dPsi, dacc, dYJ, dYY, dAm, dBm, dKd, dP, Q1, Q2;
     DevArray<double> vre, vim, dDRre, dDRim, dore, doim, dDre, dDim;
     DevArray<double> wF, wP, wN, danl, dtab, djy, dJc, dYc, dcoef;
-    DevArray<double> dAd, dAd2;
 };
 
 // One energy group: its own stream and batch buffers.  Groups run
@@ -110,7 +109,8 @@ struct sehf_scan {
     std::vector<std::unique_ptr<Group>> groups;
     int n_groups = 1;
     int ctx_device = 0;
-    int dipole_path = SEHF_DIPOLES_BACKWARD;   // how the dipole integrals are summed (sehf_scan_set_dipole_path)
+    int dipole_path = SEHF_DIPOLES_FORWARD;    // how the dipole integrals are summed (sehf_scan_set_dipole_path)
+    bool sinv_spd = true;                       // Sinv by SPD block elimination (else pivoted Gauss-Jordan)
     std::mutex host_mu;   // lazily computed host tables (stability eigenvalues)
     size_t dev_bytes_shared = 0;
     double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
@@ -376,7 +376,7 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_ck, int n_fit, bool
     b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
     b.gj2.ensure(static_cast<int>(Np), B);
     b.SinvR.alloc(2 * pp * B);
-    b.Ad.alloc(ns * Nt * B); b.Ad2.alloc(ns * Nt * B); b.Tq.alloc(ns * Np * B);
+    b.Ad.alloc(ns * Nt * B); b.Ad2.alloc(ns * Nt * B);
     if (tan) {
         const size_t no = s->S->n_occ, nx = std::max<size_t>(6, no);
         b.has_tan = true;
@@ -394,7 +394,6 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_ck, int n_fit, bool
         b.dtab.alloc(4 * Np * B); b.djy.alloc(2 * Np * B);
         b.dJc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.dYc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
         b.dcoef.alloc(4 * Np * B);
-        b.dAd.alloc(ns * Nt * B); b.dAd2.alloc(ns * Nt * B);
     }
     g.ensure_seg_events(L);
 }
@@ -420,10 +419,11 @@ void fit_window(int Nr, int& n_fs, int& n_fe) {
 }
 
 // First segment whose start matrix is kept: every segment when the dipoles
-// are back-propagated over the whole grid, else only the segments from the
-// one holding the fit window's first point (the only back-propagation left).
-int ckpt_lo(const sehf_scan* s, int L) {
-    if (s->dipole_path == SEHF_DIPOLES_BACKWARD) return 0;
+// are back-propagated over the whole grid (and in tangent runs, which always
+// back-propagate), else only the segments from the one holding the fit
+// window's first point (the only back-propagation left).
+int ckpt_lo(const sehf_scan* s, int L, bool tan) {
+    if (s->dipole_path == SEHF_DIPOLES_BACKWARD || tan) return 0;
     int n_fs = 0, n_fe = 0;
     fit_window(s->Nr, n_fs, n_fe);
     return n_fs / L;
@@ -691,12 +691,18 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // Sinv_n on the side stream (its own Gauss-Jordan workspace): it needs
     // only the shared tables and this batch's energies, so it runs ahead of
     // the forward recursion that consumes it.
+    // S_n is symmetric positive definite (the stability shift below n = 60,
+    // A_n within a few percent of I beyond), so by default it is inverted in
+    // place by block elimination without pivoting; SEHF_SINV_INVERSE=gj
+    // selects the pivoted Gauss-Jordan inverse instead.
     auto sinv_step = [&](int n, double* Sout) {
         const double* a = (n < s->nstab) ? b.dA.p + static_cast<size_t>(n) * nb : b.zeros.p;
         const double* d = (n < s->nstab) ? b.dS.p + static_cast<size_t>(n) * nb : b.zeros.p;
-        sinv_matrix(b.Ms.p, s->S0.p + static_cast<size_t>(n) * pp, s->cvd.p + static_cast<size_t>(n) * Np,
+        double* Sm = s->sinv_spd ? Sout : b.Ms.p;
+        sinv_matrix(Sm, s->S0.p + static_cast<size_t>(n) * pp, s->cvd.p + static_cast<size_t>(n) * Np,
                     s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st2);
-        gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
+        if (s->sinv_spd) spd_inverse(Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
+        else gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
     };
     auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout) {
         init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
@@ -776,15 +782,23 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     const int n_first = seg >= 0 ? std::max(ns0, seg * L) : ns0;
     // Checkpoint sg (the start matrix Rinv_{sg L - 1} of segment sg) for
     // sg >= ck_lo; a segment cycle keeps its one start matrix in slot 0.
-    const int ck_lo = ckpt_lo(s, L);
+    const int ck_lo = ckpt_lo(s, L, tan);
     auto ckp = [&](double* base, int sg) {
         const int i = seg >= 0 ? 0 : sg - ck_lo;
         return base + static_cast<size_t>(i) * nb * tt;
     };
-    // Below n_start the ratio matrices are the analytic diagonal; a whole scan
-    // with forward-accumulated dipoles still needs those points' Sinv_n for
-    // the dipole rows, so its forward loop starts at n = 1.
-    const bool adj = s->dipole_path == SEHF_DIPOLES_FORWARD;
+    // Forward accumulation of the dipole integrals (plain runs).  psi_n = P_n Z_n
+    // with P_n = [Sinv_n | X_n] (X_n = -Sinv_n B D^{-1}, zero from n_tr on) and
+    // Z_n = Rinv_n ... Rinv_{N-1} Z_N, so sum_{n<N} rows_n psi_n = Ad_{N-1} Z_N
+    // with Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n, Ad_{-1} = 0 (Rinv_0 = 0 makes
+    // Ad_0 = 0); Ad is (6 + n_occ) x N_tot per energy (ld ns).  Tangent runs
+    // back-propagate instead: the tangent of these sums, dAd_n = (dAd_{n-1} +
+    // d(rows_n P_n) + Ad_n H_n) Rinv_n, carries Rinv_n twice where a ratio
+    // matrix passes near a pole (|dRinv_n| ~ |Rinv_n|^2), while the
+    // back-propagated dZ_n = Rinv_n (dZ_{n+1} + H_n Z_n) carries it once.
+    // Below n_start the ratio matrices are the analytic diagonal; the dipole
+    // rows there still need those points' Sinv_n, so the loop starts at n = 1.
+    const bool adj = s->dipole_path == SEHF_DIPOLES_FORWARD && !tan;
     const int n_lo = (adj && seg < 0 && probe == 0) ? std::min(1, n_first) : n_first;
     auto sinv_ahead = [&](int n) {
         const int slot = n & 1;
@@ -794,55 +808,27 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     };
     const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
                                  : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
-    // Forward accumulation of the dipole integrals.  psi_n = P_n Z_n with
-    // P_n = [Sinv_n | X_n] (X_n = -Sinv_n B D^{-1}, zero from n_tr on) and
-    // Z_n = Rinv_n ... Rinv_{N-1} Z_N, so sum_{n<N} rows_n psi_n = Ad_{N-1} Z_N
-    // with Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n, Ad_{-1} = 0 (Rinv_0 = 0 makes
-    // Ad_0 = 0).  Its tangent, with dRinv_n = Rinv_n H_n Rinv_n for n >= n_start
-    // (H_n = dRinv_{n-1} + wF_n G G^T, what frp_tangent leaves in its H) and
-    // d(rows_n P_n) = wP_n (rows_n Sinv_n) P_n:
-    //   dAd_n = (dAd_{n-1} + wP_n T_n P_n + Ad_n H_n) Rinv_n,  T_n = rows_n Sinv_n;
-    // below n_start, where Rinv_n is the analytic diagonal,
-    //   dAd_n = (dAd_{n-1} + wP_n T_n P_n) Rinv_n + (Ad_{n-1} + rows_n P_n) dRinv_n.
-    // Ad, dAd are (6 + n_occ) x N_tot per energy (ld ns).
     const int64_t sAd = static_cast<int64_t>(ns) * Nt;
     auto gm = [&](double* C, int64_t ldc, int64_t sC, const double* A, int64_t lda, int64_t sA, const double* Bm,
-                  int64_t ldb, int64_t sB, int M, int Nn, int K, double alpha, double beta, const double* alpha_b) {
+                  int64_t ldb, int64_t sB, int M, int Nn, int K, double beta) {
         Gemm q;
         q.M = M; q.N = Nn; q.K = K;
         q.A = A; q.lda = lda; q.sA = sA; q.B = Bm; q.ldb = ldb; q.sB = sB;
-        q.C = C; q.ldc = ldc; q.sC = sC; q.alpha = alpha; q.beta = beta; q.alpha_b = alpha_b; q.batch = nb;
+        q.C = C; q.ldc = ldc; q.sC = sC; q.beta = beta; q.batch = nb;
         gemm(q, st);
     };
-    // Ad_{n-1} += rows_n P_n (and T_n = rows_n Sinv_n kept for the tangent)
-    auto adj_rows = [&](int n, const double* Sv) {
-        if (n >= s->n_pts) return;
-        const double* rows = s->Drows.p + static_cast<size_t>(n) * ns * Np;
-        if (tan) {
-            gm(b.Tq.p, ns, static_cast<int64_t>(ns) * Np, rows, ns, 0, Sv, Np, pp, ns, Np, Np, 1.0, 0.0, nullptr);
-            axpy_b(b.Ad.p, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, nullptr, 1.0, ns, Np, nb, st);
-        } else {
-            gm(b.Ad.p, ns, sAd, rows, ns, 0, Sv, Np, pp, ns, Np, Np, 1.0, 1.0, nullptr);
+    // Ad_{n-1} += rows_n P_n, then Ad_n = Ad_{n-1} Rinv_n
+    auto adj_step = [&](int n, const double* Sv, const double* Rn) {
+        if (n < s->n_pts) {
+            const double* rows = s->Drows.p + static_cast<size_t>(n) * ns * Np;
+            gm(b.Ad.p, ns, sAd, rows, ns, 0, Sv, Np, pp, ns, Np, Np, 1.0);
+            if (n < s->n_tr && Nf > 0)
+                gm(b.Ad.p + static_cast<size_t>(ns) * Np, ns, sAd, rows, ns, 0, Xd, Np, sXd, ns, Nf, Np, 1.0);
         }
-        if (n < s->n_tr && Nf > 0)
-            gm(b.Ad.p + static_cast<size_t>(ns) * Np, ns, sAd, rows, ns, 0, Xd, Np, sXd, ns, Nf, Np, 1.0, 1.0, nullptr);
+        gm(b.Ad2.p, ns, sAd, b.Ad.p, ns, sAd, Rn, Nt, tt, ns, Nt, Nt, 0.0);
+        std::swap(b.Ad.p, b.Ad2.p);
     };
-    // dAd_{n-1} += wP_n T_n P_n
-    auto adj_drows = [&](int n, const double* Sv) {
-        if (n >= s->n_pts) return;
-        const double* wPn = b.wP.p + static_cast<size_t>(n) * nb;
-        gm(b.dAd.p, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, Sv, Np, pp, ns, Np, Np, 1.0, 1.0, wPn);
-        if (n < s->n_tr && Nf > 0)
-            gm(b.dAd.p + static_cast<size_t>(ns) * Np, ns, sAd, b.Tq.p, ns, static_cast<int64_t>(ns) * Np, Xd, Np, sXd,
-               ns, Nf, Np, 1.0, 1.0, wPn);
-    };
-    auto adj_times_R = [&](double* out, const double* in, const double* Rn, double beta) {
-        gm(out, ns, sAd, in, ns, sAd, Rn, Nt, tt, ns, Nt, Nt, 1.0, beta, nullptr);
-    };
-    if (adj) {
-        fill(b.Ad.p, static_cast<size_t>(sAd) * nb, 0.0, st);
-        if (tan) fill(b.dAd.p, static_cast<size_t>(sAd) * nb, 0.0, st);
-    }
+    if (adj) fill(b.Ad.p, static_cast<size_t>(sAd) * nb, 0.0, st);
     if (n_lo <= n_last) sinv_ahead(n_lo);
     for (int n = n_lo; n <= n_last; ++n) {
         if (n + 1 <= n_last) sinv_ahead(n + 1);
@@ -850,40 +836,24 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
         const double* Sslot = b.SinvR.p + static_cast<size_t>(slot) * pp * nb;
         if (n < ns0) {
-            // analytic region: Rinv_n = diag(anl_n) built in Rc (Rp holds the
-            // start matrix Rinv_{n_start-1} and is not touched)
-            if (n < s->n_tr && Nf > 0)
-                gm(Xd, Np, sXd, Sslot, Np, pp, s->BD.p + static_cast<size_t>(n) * pf, Np, 0, Np, Nf, Np, -1.0, 0.0,
-                   nullptr);
-            adj_rows(n, Sslot);
-            diag_matrix(b.Rc.p, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
-            adj_times_R(b.Ad2.p, b.Ad.p, b.Rc.p, 0.0);
-            if (tan) {
-                diag_matrix(b.dRc.p, b.danl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
-                adj_drows(n, Sslot);
-                adj_times_R(b.dAd2.p, b.dAd.p, b.Rc.p, 0.0);
-                adj_times_R(b.dAd2.p, b.Ad.p, b.dRc.p, 1.0);
-                std::swap(b.dAd.p, b.dAd2.p);
+            // analytic region (forward-accumulated dipoles only): Rinv_n =
+            // diag(anl_n) built in Rc; Rp holds the start matrix Rinv_{n_start-1}
+            if (n < s->n_tr && Nf > 0) {
+                Gemm g;   // Xn = -Sinv (B D^{-1})
+                g.M = Np; g.N = Nf; g.K = Np;
+                g.A = Sslot; g.lda = Np; g.sA = pp;
+                g.B = s->BD.p + static_cast<size_t>(n) * pf; g.ldb = Np; g.sB = 0;
+                g.C = Xd; g.ldc = Np; g.sC = sXd; g.alpha = -1.0; g.batch = nb;
+                gemm(g, st);
             }
-            std::swap(b.Ad.p, b.Ad2.p);
+            diag_matrix(b.Rc.p, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
+            adj_step(n, Sslot, b.Rc.p);
             SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
             continue;
         }
         frp_step(n, Sslot, b.Rp.p, b.Rc.p);
-        if (adj) {
-            adj_rows(n, Sslot);
-            adj_times_R(b.Ad2.p, b.Ad.p, b.Rc.p, 0.0);
-        }
-        if (tan) {
-            frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);   // dRp now holds H_n
-            if (adj) {
-                adj_drows(n, Sslot);
-                gm(b.dAd.p, ns, sAd, b.Ad2.p, ns, sAd, b.dRp.p, Nt, tt, ns, Nt, Nt, 1.0, 1.0, nullptr);   // += Ad_n H_n
-                adj_times_R(b.dAd2.p, b.dAd.p, b.Rc.p, 0.0);
-                std::swap(b.dAd.p, b.dAd2.p);
-            }
-        }
-        if (adj) std::swap(b.Ad.p, b.Ad2.p);
+        if (adj) adj_step(n, Sslot, b.Rc.p);
+        if (tan) frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
         std::swap(b.Rp.p, b.Rc.p);
         if (tan) std::swap(b.dRp.p, b.dRc.p);
@@ -1050,16 +1020,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (N < s->n_pts && seg < 0) dip_accum(N, b.dP.p, b.dacc.p);
         if (N >= n_fs && N <= n_fe && seg < 0) fit_tangent(N, b.P.p, b.dP.p);
     }
-    if (adj && seg < 0) {
-        // the forward-accumulated rows applied to the outer boundary:
-        // sum_{n<N} rows_n psi_n = Ad_{N-1} Z_N, its tangent dAd Z_N + Ad dZ_N
-        const int64_t sZN = static_cast<int64_t>(Nt) * Np, sAc = static_cast<int64_t>(ns) * Np;
-        gm(b.acc.p, ns, sAc, b.Ad.p, ns, sAd, b.Z.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
-        if (tan) {
-            gm(b.dacc.p, ns, sAc, b.dAd.p, ns, sAd, b.Z.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
-            gm(b.dacc.p, ns, sAc, b.Ad.p, ns, sAd, b.dZ.p, Nt, sZN, ns, Np, Nt, 1.0, 1.0, nullptr);
-        }
-    }
+    if (adj && seg < 0)   // the forward-accumulated rows applied to the outer boundary: Ad_{N-1} Z_N
+        gm(b.acc.p, ns, static_cast<int64_t>(ns) * Np, b.Ad.p, ns, sAd, b.Z.p, Nt, static_cast<int64_t>(Nt) * Np, ns,
+           Np, Nt, 1.0);
     fill(b.red.p, 2 * nb, 0.0, st);
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
     const double tw3 = wall();
@@ -1094,8 +1057,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // Three kinds of pass, each over segments sg_hi down to sg_lo with every
     // segment's ratio matrices recomputed from its checkpoint:
     //   FitSums: from the top down to the fit window's first point n_fs, psi
-    //     (and, with forward-accumulated dipoles, the tangent) into the
-    //     window sums;
+    //     into the window sums;
     //   Resid:   the same points again once A, B are known, the fit residual;
     //   Full:    the whole grid with the dipole sums, the fit residual and the
     //     tangent (back-propagated dipoles; a segment cycle's pass).
@@ -1109,7 +1071,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         for (int sg = sg_hi; sg >= sg_lo; --sg) {
             const int n0 = sg * L, n1 = std::min(n0 + L, N);
             const double* prev = (sg == 0) ? nullptr : ckp(b.ckpt.p, sg);
-            const bool tg = tan && (main || (kind == Pass::FitSums && adj));
+            const bool tg = tan && main;
             const double* dprev0 = (sg == 0 || !tg) ? nullptr : ckp(b.dckpt.p, sg);
             const double* dprev = dprev0;
             // the segment's Schur inverses on the side stream, after the
@@ -1406,6 +1368,9 @@ int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
         s->st = sehf_ctx_stream(ctx);
         s->ctx_device = sehf_ctx_device(ctx);
         if (const char* e = std::getenv("SEHF_SCAN_GROUPS")) s->n_groups = std::max(1, std::min(4, std::atoi(e)));
+        if (const char* e = std::getenv("SEHF_SINV_INVERSE")) s->sinv_spd = std::string(e) != "gj";
+        if (const char* e = std::getenv("SEHF_DIPOLE_PATH"))
+            s->dipole_path = std::atoi(e) == SEHF_DIPOLES_BACKWARD ? SEHF_DIPOLES_BACKWARD : SEHF_DIPOLES_FORWARD;
         s->S = std::make_unique<Setup>(*p);
         const Setup& S = *s->S;
         s->Np = S.Np; s->Nf = S.Nf; s->Nt = S.Nt; s->Nr = S.Nr; s->n_tr = S.n_tr; s->n_start = S.n_start;
@@ -1449,7 +1414,8 @@ void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batc
         s->groups.emplace_back(new Group());
         s->groups.back()->init();
     }
-    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L), n_fit, dir != nullptr);
+    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L, dir != nullptr), n_fit,
+                                                 dir != nullptr);
     if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
     // batch i runs on group i % G; each group is driven by its own host thread
     std::vector<std::string> errs(G);
@@ -1500,7 +1466,7 @@ void scan_segment(sehf_scan* s, const double* k, const double* dir, int nk, int 
         s->groups.back()->init();
     }
     SEHF_HIP_CHECK(hipSetDevice(s->ctx_device));
-    alloc_batch(s, *s->groups[0], nk, L, n_seg - ckpt_lo(s, L), 200, dir != nullptr);
+    alloc_batch(s, *s->groups[0], nk, L, n_seg - ckpt_lo(s, L, dir != nullptr), 200, dir != nullptr);
     double tt[4] = {0, 0, 0, 0};
     run_batch(s, *s->groups[0], k, dir, nk, L, nullptr, nullptr, 0, 0, nullptr, seg, tt);
     if (t) std::copy(tt, tt + 4, t);
@@ -1551,7 +1517,7 @@ int sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, i
                     s->groups.back()->init();
                 }
                 for (int gi = 0; gi < G; ++gi)
-                    alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L), n_fit, tangent != 0);
+                    alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L, tangent != 0), n_fit, tangent != 0);
             } catch (const std::exception&) {
                 (void)hipGetLastError();
                 return -1.0;

@@ -133,6 +133,52 @@ int sehf_time_inverse(int n, int batch, int reps, double* ms_out, double* resid_
     });
 }
 
+int sehf_time_spd_inverse(int n, int batch, int reps, double* ms_out, double* resid_out) {
+    return guarded_t([&] {
+        if (n <= 0 || batch <= 0 || reps <= 0)
+            throw std::runtime_error("sehf_time_spd_inverse: n, batch and reps must be > 0");
+        const size_t per = static_cast<size_t>(n) * n;
+        // I + 0.05 (R + R^T) / sqrt(n), R uniform in [-0.5, 0.5): symmetric
+        // positive definite with condition ~1.1, like the Schur matrices
+        std::vector<double> r(per), h(per * batch);
+        lcg_fill(r, 777);
+        const double sc = 0.05 / std::sqrt(static_cast<double>(n));
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                h[i + static_cast<size_t>(j) * n] = (i == j ? 1.0 : 0.0) +
+                                                    sc * (r[i + static_cast<size_t>(j) * n] + r[j + static_cast<size_t>(i) * n]);
+        for (int bb = 1; bb < batch; ++bb) std::copy(h.begin(), h.begin() + per, h.begin() + bb * per);
+        DevArray<double> A(per * batch), W(per * batch);
+        SEHF_HIP_CHECK(hipMemcpy(A.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        GjBuffers gb;
+        gb.ensure(n, batch);
+        double total = 0.0;
+        for (int rep = 0; rep <= reps; ++rep) {   // rep 0 warms up
+            SEHF_HIP_CHECK(hipMemcpy(W.p, A.p, per * batch * sizeof(double), hipMemcpyDeviceToDevice));
+            EventPair ev;
+            SEHF_HIP_CHECK(hipEventRecord(ev.a, nullptr));
+            spd_inverse(W.p, n, n, static_cast<int64_t>(per), batch, true, gb.work(), nullptr);
+            SEHF_HIP_CHECK(hipEventRecord(ev.b, nullptr));
+            const double ms = ev.ms();
+            if (rep > 0) total += ms;
+        }
+        check_gj_info(gb, batch, nullptr, "sehf_time_spd_inverse");
+        if (resid_out) {   // sampled max |A X - I| of the last matrix
+            std::vector<double> x(per);
+            SEHF_HIP_CHECK(hipMemcpy(x.data(), W.p + (batch - 1) * per, per * sizeof(double), hipMemcpyDeviceToHost));
+            double worst = 0.0;
+            for (int j = 0; j < n; j += 7)
+                for (int i = 0; i < n; i += 5) {
+                    double acc = 0.0;
+                    for (int q = 0; q < n; ++q) acc += h[i + static_cast<size_t>(q) * n] * x[q + static_cast<size_t>(j) * n];
+                    worst = std::max(worst, std::fabs(acc - (i == j ? 1.0 : 0.0)));
+                }
+            *resid_out = worst;
+        }
+        *ms_out = total / reps;
+    });
+}
+
 int sehf_copy_probe(size_t bytes, double* ms_out) {
     return guarded_t([&] {
         if (bytes == 0) throw std::runtime_error("sehf_copy_probe: bytes must be > 0");

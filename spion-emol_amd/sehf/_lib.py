@@ -65,6 +65,7 @@ SIGNATURES = [
     ("sehf_prof_end", ctypes.c_int, [_dp]),
     ("sehf_time_gemm", ctypes.c_int, [ctypes.c_int] * 9 + [_dp]),
     ("sehf_time_inverse", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    ("sehf_time_spd_inverse", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     ("sehf_copy_probe", ctypes.c_int, [ctypes.c_size_t, _dp]),
     ("sehf_host_potential", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("sehf_host_exchange_B", ctypes.c_int, [_vp, ctypes.c_int, _dp]),

@@ -138,10 +138,11 @@ class Scan:
     DIPOLES_BACKWARD = 1   # SEHF_DIPOLES_BACKWARD
 
     def set_dipole_path(self, path: int):
-        """How the dipole integrals are summed: DIPOLES_FORWARD carries the adjoint row block Ad_n = (Ad_{n-1} + rows_n P_n) Rinv_n
-        along the forward recursion and back-propagates only over the fit
-        window; DIPOLES_BACKWARD (default) back-propagates over the whole
-        grid (sehf_scan_set_dipole_path)."""
+        """How plain runs sum the dipole integrals: DIPOLES_FORWARD
+        (default) carries the adjoint row block Ad_n = (Ad_{n-1} + rows_n P_n)
+        Rinv_n along the forward recursion and back-propagates only over the
+        fit window; DIPOLES_BACKWARD back-propagates over the whole grid.
+        Tangent runs always back-propagate (sehf_scan_set_dipole_path)."""
         check(load().sehf_scan_set_dipole_path(self._h, int(path)), "sehf_scan_set_dipole_path")
 
     def layout(self, seg_len: int = 0) -> dict:

@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+from decimal import Decimal, localcontext
 
 import numpy as np
 
@@ -90,14 +91,103 @@ class Problem:
 
 
 # ---------------------------------------------------------------------------
+# Host-independent arithmetic.  The generated inputs are bit for bit the same
+# on every host: full-size oracle fixtures computed on one machine are
+# compared with device runs whose inputs are generated on another, and the
+# energy-amplified outputs move by 1e-8 under a last-bit change of the
+# inputs.  So nothing below goes through libm transcendentals, BLAS or SIMD
+# reductions, whose last bits depend on the CPU model and its vector width:
+# only elementwise IEEE + - * / and sqrt (correctly rounded), sequential
+# accumulation in a fixed order, math.fsum (correctly rounded sums) and
+# decimal arithmetic (software, correctly rounded conversions).
+# ---------------------------------------------------------------------------
+_PI_DEC = Decimal("3.14159265358979323846264338327950288419716939937510582097494459230781640628620899863")
+_PREC = 40
+
+
+def _dec_cos_sin(x: Decimal) -> tuple:
+    """cos x, sin x by their Taylor series at 40 significant digits."""
+    with localcontext() as c:
+        c.prec = _PREC + 10
+        two_pi = 2 * _PI_DEC
+        x = x % two_pi
+        if x > _PI_DEC:
+            x -= two_pi
+        x2, term, cs, k = x * x, Decimal(1), Decimal(1), 0
+        while True:
+            k += 2
+            term = -term * x2 / (k * (k - 1))
+            if term == 0 or abs(term) < Decimal(10) ** -(_PREC + 5):
+                break
+            cs += term
+        term, sn, k = x, x, 1
+        while True:
+            k += 2
+            term = -term * x2 / (k * (k - 1))
+            if term == 0 or abs(term) < Decimal(10) ** -(_PREC + 5):
+                break
+            sn += term
+        return +cs, +sn
+
+
+def _gauss_legendre(n: int) -> tuple:
+    """Gauss-Legendre nodes (ascending) and weights on [-1, 1]: Newton on the
+    three-term recurrence of P_n in decimal arithmetic, then rounded once."""
+    xs, ws = [], []
+    with localcontext() as c:
+        c.prec = _PREC + 10
+        tol = Decimal(10) ** -(_PREC + 2)
+        for i in range(1, n + 1):
+            x, _ = _dec_cos_sin(_PI_DEC * (Decimal(i) - Decimal("0.25")) / (Decimal(n) + Decimal("0.5")))
+            for _ in range(200):
+                p0, p1 = Decimal(1), x
+                for k in range(2, n + 1):
+                    p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+                dp = n * (x * p1 - p0) / (x * x - 1)
+                dx = p1 / dp
+                x -= dx
+                if abs(dx) < tol:
+                    break
+            p0, p1 = Decimal(1), x
+            for k in range(2, n + 1):
+                p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+            dp = n * (x * p1 - p0) / (x * x - 1)
+            xs.append(float(x))
+            ws.append(float(2 / ((1 - x * x) * dp * dp)))
+    order = np.argsort(xs)
+    return np.array(xs)[order], np.array(ws)[order]
+
+
+def _exp_neg(zeta: float, dr: float, Nr: int) -> np.ndarray:
+    """e^{-zeta n dr} for n in [0, Nr): e^{-64 a zeta dr} e^{-b zeta dr} with
+    n = 64 a + b, both factors from decimal exp (one rounding each), so every
+    value is within an ulp or two of the exact one and the same on every host."""
+    with localcontext() as c:
+        c.prec = _PREC
+        zd = Decimal(zeta) * Decimal(dr)
+        lo = np.array([float((-(zd * b)).exp()) for b in range(64)])
+        hi = np.array([float((-(zd * 64 * a)).exp()) for a in range((Nr + 63) // 64)])
+    n = np.arange(Nr)
+    return hi[n // 64] * lo[n % 64]
+
+
+def _ipow(r: np.ndarray, k: int) -> np.ndarray:
+    """r^k for an integer k >= 0 by repeated multiplication."""
+    out = np.ones_like(r)
+    for _ in range(k):
+        out = out * r
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Real spherical harmonics on an angular grid (vectorised).
 # ---------------------------------------------------------------------------
-def real_ylm_table(lmax: int, theta: np.ndarray, phi: np.ndarray) -> np.ndarray:
-    """Y^R_{lm}(theta, phi) for all (l, m), l <= lmax; shape [(lmax+1)^2, npts]."""
-    ct, st = np.cos(theta), np.sin(theta)
-    out = np.zeros(((lmax + 1) ** 2, theta.size))
+def _ylm_core(lmax: int, ct: np.ndarray, st: np.ndarray, cos_m, sin_m) -> np.ndarray:
+    """Y^R_{lm} from cos(theta), sin(theta) and cos(m phi), sin(m phi) tables;
+    shape [(lmax+1)^2, npts]."""
+    out = np.zeros(((lmax + 1) ** 2, ct.size))
     # normalised associated Legendre S_l^m without Condon-Shortley phase
-    s_mm = np.full(theta.shape, 1.0 / math.sqrt(4.0 * math.pi))
+    s_mm = np.full(ct.shape, 1.0 / math.sqrt(4.0 * math.pi))
     for m in range(lmax + 1):
         if m > 0:
             s_mm = s_mm * math.sqrt((2.0 * m + 1.0) / (2.0 * m)) * st
@@ -117,21 +207,44 @@ def real_ylm_table(lmax: int, theta: np.ndarray, phi: np.ndarray) -> np.ndarray:
             if m == 0:
                 out[l * l + l] = s
             else:
-                out[l * l + l + m] = math.sqrt(2.0) * s * np.cos(m * phi)
-                out[l * l + l - m] = math.sqrt(2.0) * s * np.sin(m * phi)
+                out[l * l + l + m] = math.sqrt(2.0) * s * cos_m(m)
+                out[l * l + l - m] = math.sqrt(2.0) * s * sin_m(m)
     return out
+
+
+def real_ylm_table(lmax: int, theta: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """Y^R_{lm}(theta, phi) for all (l, m), l <= lmax; shape [(lmax+1)^2, npts]."""
+    return _ylm_core(lmax, np.cos(theta), np.sin(theta), lambda m: np.cos(m * phi), lambda m: np.sin(m * phi))
 
 
 def angular_grid(lmax_integrand: int):
     """Gauss-Legendre x trapezoid grid exact for polynomials of degree <= lmax_integrand."""
     n_t = lmax_integrand // 2 + 2
     n_p = lmax_integrand + 2
-    x, w = np.polynomial.legendre.leggauss(n_t)
+    x, w = _gauss_legendre(n_t)
     theta = np.arccos(x)
     phi = 2.0 * np.pi * np.arange(n_p) / n_p
     T, P = np.meshgrid(theta, phi, indexing="ij")
     W = np.outer(w, np.full(n_p, 2.0 * np.pi / n_p))
     return T.ravel(), P.ravel(), W.ravel()
+
+
+def _ylm_on_grid(lmax: int, lmax_integrand: int):
+    """Y^R_{lm} on angular_grid(lmax_integrand)'s points and its weights, from
+    host-independent cos/sin values (decimal), in the grid's point order."""
+    n_t = lmax_integrand // 2 + 2
+    n_p = lmax_integrand + 2
+    x, w = _gauss_legendre(n_t)
+    cs = [_dec_cos_sin(2 * _PI_DEC * j / n_p) for j in range(n_p)]
+    cos_t = np.array([float(c) for c, _ in cs])
+    sin_t = np.array([float(s_) for _, s_ in cs])
+    ct = np.repeat(x, n_p)
+    st = np.sqrt((1.0 - x) * (1.0 + x))
+    st = np.repeat(st, n_p)
+    jj = np.tile(np.arange(n_p), n_t)
+    W = np.repeat(w, n_p) * (2.0 * math.pi / n_p)
+    Y = _ylm_core(lmax, ct, st, lambda m: cos_t[(m * jj) % n_p], lambda m: sin_t[(m * jj) % n_p])
+    return Y, W
 
 
 # ---------------------------------------------------------------------------
@@ -145,42 +258,52 @@ def _cumtrapz(f: np.ndarray, dr: float) -> np.ndarray:
 
 def hartree_from_orbitals(psi_lm: np.ndarray, n_occ: int, lmax_orb: int, lmax_vh: int,
                           dr: float, occ: float = 2.0) -> np.ndarray:
-    """V_H[lm, ir] for lm up to (lmax_vh+1)^2 from the closed-shell density."""
+    """V_H[lm, ir] for lm up to (lmax_vh+1)^2 from the closed-shell density.
+    The angular contractions accumulate term by term in a fixed order (no
+    BLAS), so the result is the same on every host."""
     Nr = psi_lm.shape[-1]
     l_rho = min(2 * lmax_orb, lmax_vh)
-    theta, phi, w = angular_grid(2 * lmax_orb + l_rho)
+    deg = 2 * lmax_orb + l_rho
     n_lo = (lmax_orb + 1) ** 2
-    Y_orb = real_ylm_table(lmax_orb, theta, phi)            # [n_lo, nq]
-    Y_rho = real_ylm_table(l_rho, theta, phi) * w          # [n_lr, nq] with weights
+    Y_orb, _ = _ylm_on_grid(lmax_orb, deg)                  # [n_lo, nq]
+    Y_rho, wq = _ylm_on_grid(l_rho, deg)
+    Y_rho = Y_rho * wq                                      # [n_lr, nq] with weights
+    nq = wq.size
     rho_lm = np.zeros(((l_rho + 1) ** 2, Nr))
     chunk = 512
     for r0 in range(0, Nr, chunk):
         sl = slice(r0, min(Nr, r0 + chunk))
-        dens = np.zeros((sl.stop - sl.start, theta.size))
+        dens = np.zeros((sl.stop - sl.start, nq))
         for i in range(n_occ):
-            phi_i = psi_lm[i, :n_lo, sl].T @ Y_orb              # [nr, nq]
+            phi_i = np.zeros_like(dens)
+            for lam in range(n_lo):
+                phi_i += psi_lm[i, lam, sl][:, None] * Y_orb[lam][None, :]
             dens += occ * phi_i * phi_i
-        rho_lm[:, sl] = (dens @ Y_rho.T).T
+        acc = np.zeros((Y_rho.shape[0], sl.stop - sl.start))
+        for q in range(nq):
+            acc += Y_rho[:, q][:, None] * dens[:, q][None, :]
+        rho_lm[:, sl] = acc
     r = np.arange(Nr) * dr
     V = np.zeros(((lmax_vh + 1) ** 2, Nr))
     rp = r[1:]
     for l in range(l_rho + 1):
         pref = 4.0 * math.pi / (2.0 * l + 1.0)
+        rl, rl1 = _ipow(rp, l), _ipow(rp, l + 1)
         for m in range(-l, l + 1):
             ch = l * l + l + m
             rho = rho_lm[ch]
             if np.max(np.abs(rho)) < 1e-14:
                 continue
-            f_in = rho * r ** (l + 2)
+            f_in = rho * _ipow(r, l + 2)
             f_out = np.zeros(Nr)
             if l == 0:
                 f_out = rho * r
             else:
-                f_out[1:] = rho[1:] * rp ** (1 - l)
+                f_out[1:] = rho[1:] / _ipow(rp, l - 1)
             I_in = _cumtrapz(f_in, dr)
             I_out = _cumtrapz(f_out, dr)
             v = np.zeros(Nr)
-            v[1:] = pref * (I_in[1:] / rp ** (l + 1) + rp ** l * (I_out[-1] - I_out[1:]))
+            v[1:] = pref * (I_in[1:] / rl1 + rl * (I_out[-1] - I_out[1:]))
             if l == 0:
                 v[0] = pref * I_out[-1]
             V[ch] = v
@@ -197,23 +320,27 @@ def _simpson_weights(Nr: int, dr: float) -> np.ndarray:
 
 
 def _gram_schmidt(F: np.ndarray, dr: float) -> np.ndarray:
-    """Orthonormalise orbitals F[i, lam, r] under sum_lam int F_i F_j r^2 dr."""
+    """Orthonormalise orbitals F[i, lam, r] under sum_lam int F_i F_j r^2 dr
+    (overlaps by math.fsum, correctly rounded)."""
     Nr = F.shape[-1]
     r2w = (np.arange(Nr) * dr) ** 2 * _simpson_weights(Nr, dr)
     out = F.copy()
     for i in range(out.shape[0]):
         for j in range(i):
-            ov = np.sum(out[i] * out[j] * r2w)
+            ov = math.fsum((out[i] * out[j] * r2w).ravel())
             out[i] -= ov * out[j]
-        nrm = math.sqrt(np.sum(out[i] * out[i] * r2w))
+        nrm = math.sqrt(math.fsum((out[i] * out[i] * r2w).ravel()))
         out[i] /= nrm
     return out
 
 
-def _slater(l: int, zeta: float, r: np.ndarray) -> np.ndarray:
-    """Normalised Slater radial function r^l e^{-zeta r} (int R^2 r^2 dr = 1)."""
-    norm = (2.0 * zeta) ** (l + 1.5) / math.sqrt(math.factorial(2 * l + 2))
-    return norm * r ** l * np.exp(-zeta * r)
+def _slater(l: int, zeta: float, dr: float, Nr: int) -> np.ndarray:
+    """Normalised Slater radial function r^l e^{-zeta r} (int R^2 r^2 dr = 1)
+    on r = n dr."""
+    r = np.arange(Nr) * dr
+    tz = 2.0 * zeta
+    norm = _ipow(np.array([tz]), l + 1)[0] * math.sqrt(tz) / math.sqrt(float(math.factorial(2 * l + 2)))
+    return norm * _ipow(r, l) * _exp_neg(zeta, dr, Nr)
 
 
 # ---------------------------------------------------------------------------
@@ -231,7 +358,7 @@ def h2_like(l_cont: int = 8, dr: float = 0.01, r_max: float = 30.0, seed: int = 
     F = np.zeros((1, nlm_orb, Nr))
     for (l, cl) in ((0, c[0]), (2, c[1])):
         if (l + 1) ** 2 <= nlm_orb:
-            F[0, l * l + l, :] = r * np.exp(-1.2 * r) * (1.0 + cl * r)
+            F[0, l * l + l, :] = r * _exp_neg(1.2, dr, Nr) * (1.0 + cl * r)
     F = _gram_schmidt(F, dr)
     lvh = 2 * l_cont if lmax_vh is None else lmax_vh
     V_H = hartree_from_orbitals(F, 1, min(2, l_orb), lvh, dr)
@@ -257,7 +384,7 @@ def h2o_like(l_cont: int = 20, dr: float = 0.01, r_max: float = 50.0, n_occ: int
             l = int(math.isqrt(lam))
             zetas = rng.uniform(0.8, 8.0, size=3)
             coefs = rng.normal(0.0, 1.0, size=3)
-            F[i, lam] = sum(cf * _slater(l, z, r) for cf, z in zip(coefs, zetas))
+            F[i, lam] = sum(cf * _slater(l, z, dr, Nr) for cf, z in zip(coefs, zetas))
     F = _gram_schmidt(F, dr)
     V_H = hartree_from_orbitals(F, n_occ, lsce, 2 * l_cont, dr)
     energies = np.array([-20.55, -1.34, -0.70, -0.57, -0.50][:n_occ]) if n_occ <= 5 else \
@@ -275,8 +402,21 @@ def k_grid(k_min: float, k_max: float, n: int) -> np.ndarray:
     return np.linspace(k_min, k_max, n)
 
 
-# The benchmark configurations (grid, cutoffs and scans).
+_CONFIG_CACHE: dict = {}
+
+
+# The benchmark configurations (grid, cutoffs and scans).  Generating one
+# takes tens of seconds (host-independent arithmetic), so each is built once
+# per process; callers get their own copy of the arrays.
 def config(name: str) -> tuple[Problem, np.ndarray]:
+    if name not in _CONFIG_CACHE:
+        _CONFIG_CACHE[name] = _build_config(name)
+    p, ks = _CONFIG_CACHE[name]
+    return dataclasses.replace(p, V_H=p.V_H.copy(), psi_lm=p.psi_lm.copy(), xyz=p.xyz.copy(), Z=p.Z.copy(),
+                               orb_energies=p.orb_energies.copy()), ks.copy()
+
+
+def _build_config(name: str) -> tuple[Problem, np.ndarray]:
     if name == "h2_l8":          # CPU-runnable plain case
         return h2_like(8, 0.01, 30.0, seed=0), k_grid(0.1, 1.0, 10)
     if name == "h2o_l20":        # plain per-ik on one GPU

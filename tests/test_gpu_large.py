@@ -1,0 +1,240 @@
+"""Full-size parity: the device scan against the CPU oracle's fixtures of the
+l_cont = 20 and 30 configurations (tests/golden/large_*.npz, written by
+tests/golden/make_large_fixtures.py), and full-size properties of the device
+results that the mathematics fixes.
+
+* The integer diagnostics (n_transition, n_start, fit window, shift counts,
+  closed-l count) exactly, on every fixture energy.
+* Every output (d_lm, K's diagonal and three full rows, the diagonals of
+  K_fit, A and B, d_raw, b_overlap) against the referee: the oracle with
+  correctly rounded inverses (every dense inverse by 80-bit LU rounded once,
+  all else the FP64 reference algorithm; the *_mode32 fixtures), within
+  max(1e-9, the FP64 oracle's own distance to it).  At full size the FP64
+  algorithm's result moves under a change of rounding order by far more than
+  1e-9: energy enters the Schur matrix only as (h^2/6) E ~ 1e-5 .. 1e-7 on an
+  O(1) diagonal, so eps-level rounding acts as an energy jitter, which narrow
+  high-l resonances amplify; two rounding orders of the device path itself
+  (Schur inverses by block elimination or by pivoted Gauss-Jordan) differ by
+  4e-8 at l_cont = 20, k = 1.1.  The device must be at least as accurate as
+  the reference's FP64 algorithm.  Fixture energies without a referee are
+  compared for the integer diagnostics only (the float distances are
+  printed).
+* Tangent fixtures (the adjoint configuration h2o_l20_pol along E and along
+  alpha): dK's diagonal, dd_raw and dD against the tangent referee where
+  present, within max(1e-8, the FP64 oracle's distance to it).
+* Properties: S = (1 + iK)(1 - iK)^{-1} unitary; on an energy where every
+  channel is open, K_fit from the asymptotic fit is symmetric and agrees with
+  the two-point K to the level the fit window's neglected potential tail
+  allows; at
+  the adjoint configuration's size the analytic Wigner delay from dd/dE
+  agrees with the finite-difference-in-k delay of the reference's
+  post-processing (numpy.gradient over k, cross_section_delay.py:227-263)
+  extrapolated to zero spacing.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _fixtures():
+    """{(config, tangent): [(FP64 fixture, referee or None), ...]} by energy."""
+    fp64, ref = {}, {}
+    for f in sorted(glob.glob(os.path.join(GOLD, "large_*.npz"))):
+        d = np.load(f, allow_pickle=False)
+        cfg = str(d["config"])
+        tan = tuple(float(x) for x in d["tangent"]) if "tangent" in d.files else None
+        mode = int(d["inverse_mode"]) if "inverse_mode" in d.files else 0
+        (ref if mode == 32 else fp64)[(cfg, tan, int(d["ik"]))] = d
+    out = {}
+    for (cfg, tan, ik), d in sorted(fp64.items()):
+        out.setdefault((cfg, tan), []).append((d, ref.get((cfg, tan, ik))))
+    return out
+
+
+FIX = _fixtures()
+PLAIN = sorted({c for (c, t) in FIX if t is None})
+TANG = sorted((c, t) for (c, t) in FIX if t is not None)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from sehf.steppers import GpuContext
+    return GpuContext(True)
+
+
+def _run(ctx, p, ks, tangent=None, sinv=None):
+    from sehf import scan
+    old = os.environ.get("SEHF_SINV_INVERSE")
+    if sinv is not None:
+        os.environ["SEHF_SINV_INVERSE"] = sinv
+    try:
+        S = scan.Scan(p, ctx)
+    finally:
+        if sinv is not None:
+            if old is None:
+                del os.environ["SEHF_SINV_INVERSE"]
+            else:
+                os.environ["SEHF_SINV_INVERSE"] = old
+    try:
+        return S.run(ks, batch=len(ks), tangent=tangent)
+    finally:
+        S.close()
+
+
+def _view(r, rows):
+    """The device result in the fixture's compact form."""
+    return {"D": r["D"], "d_raw": r["d_raw"], "b_overlap": r["b_overlap"], "K_diag": np.diag(r["K"]),
+            "K_rows": r["K"][rows], "Kfit_diag": np.diag(r["K_fit"]), "A_diag": np.diag(r["A"]),
+            "B_diag": np.diag(r["B"])}
+
+
+KEYS = ("D", "K_diag", "K_rows", "Kfit_diag", "A_diag", "B_diag", "d_raw", "b_overlap")
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("config", PLAIN)
+def test_full_size_fixture_parity(ctx, config):
+    from sehf import synth
+    p, ks_all = synth.config(config)
+    fx = FIX[(config, None)]
+    ks = np.array([float(d["k"]) for d, _ in fx])
+    for d, _ in fx:
+        assert float(ks_all[int(d["ik"])]) == float(d["k"])
+    got = _run(ctx, p, ks)
+    probe = _run(ctx, p, ks, sinv="gj")
+    fails, refereed = [], 0
+    for e_i, ((d, x), g) in enumerate(zip(fx, got)):
+        rows = d["K_row_idx"]
+        gv, qv = _view(g, rows), _view(probe[e_i], rows)
+        line = []
+        for key in KEYS:
+            e = rel_err(gv[key], d[key])
+            fl = rel_err(gv[key], qv[key])
+            if x is None:
+                line.append(f"{key}={e:.1e}(device orders {fl:.1e})")
+                continue
+            ex = rel_err(gv[key], x[key])
+            bound = max(1e-9, rel_err(d[key], x[key]))
+            line.append(f"{key}={ex:.1e}(<{bound:.1e}; FP64 oracle {e:.1e}, device orders {fl:.1e})")
+            if not ex < bound:
+                fails.append((float(d["k"]), key, ex, bound))
+        refereed += x is not None
+        for idx in (4, 5, 6, 7, 8, 9, 10):
+            if g["diag"][idx] != d["diag"][idx]:
+                fails.append((float(d["k"]), f"diag[{idx}]", g["diag"][idx], d["diag"][idx]))
+        print(f"  {config} k={float(d['k']):.4f} " + ("against the referee: " if x is not None else
+                                                       "(no referee fixture): ") + " ".join(line))
+    assert not fails, (config, fails)
+    print(f"  {config}: {refereed} of {len(fx)} energies refereed")
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("case", TANG, ids=[f"{c}_tan{t[0]:g}_{t[1]:g}" for c, t in TANG])
+def test_full_size_tangent_fixture(ctx, case):
+    from sehf import synth
+    config, tan = case
+    p, _ = synth.config(config)
+    fx = FIX[case]
+    ks = np.array([float(d["k"]) for d, _ in fx])
+    got = _run(ctx, p, ks, tangent=tan)
+    probe = _run(ctx, p, ks, tangent=tan, sinv="gj")
+
+    def tv(r):
+        return {"D": r["D"], "dD": r["dD"], "dd_raw": r["dd_raw"], "dK_diag": np.diag(r["dK"])}
+    fails = []
+    for e_i, ((d, x), g) in enumerate(zip(fx, got)):
+        gv, qv = tv(g), tv(probe[e_i])
+        line = []
+        for key in ("D", "dD", "dd_raw", "dK_diag"):
+            e = rel_err(gv[key], d[key])
+            fl = rel_err(gv[key], qv[key])
+            if x is None:
+                line.append(f"{key}={e:.1e}(device orders {fl:.1e})")
+                continue
+            ex = rel_err(gv[key], x[key])
+            bound = max(1e-9 if key == "D" else 1e-8, rel_err(d[key], x[key]))
+            line.append(f"{key}={ex:.1e}(<{bound:.1e}; FP64 oracle {e:.1e}, device orders {fl:.1e})")
+            if not ex < bound:
+                fails.append((key, ex, bound))
+        for idx in (4, 5, 6, 7, 8, 9, 10):
+            if g["diag"][idx] != d["diag"][idx]:
+                fails.append((f"diag[{idx}]", g["diag"][idx], d["diag"][idx]))
+        print(f"  {config} k={float(d['k']):.4f} tangent {tan} " +
+              ("against the referee: " if x is not None else "(no referee fixture): ") + " ".join(line))
+    assert not fails, (config, tan, fails)
+
+
+def _unitarity(K):
+    n = K.shape[0]
+    S = (np.eye(n) + 1j * K) @ np.linalg.inv(np.eye(n) - 1j * K)
+    return float(np.abs(S @ S.conj().T - np.eye(n)).max())
+
+
+@pytest.mark.timeout(900)
+def test_full_size_properties_l20(ctx):
+    """h2o_l20 at k = 1.1 (kr = 55 > l_cont at the boundary: every channel
+    open) and k = 0.1 (closed channels): S from the two-point K is unitary at
+    both; where every channel is open, K_fit from the asymptotic fit is
+    symmetric to the fit's accuracy and agrees with the two-point K.  (A
+    closed l is fitted as A = 1, B = 0, AsymptoticAmplitudes.cpp, so K_fit is
+    less symmetric there.)"""
+    from sehf import synth
+    p, ks_all = synth.config("h2o_l20")
+    ks = np.array([ks_all[0], ks_all[52]])
+    res = _run(ctx, p, ks)
+    stats = []
+    for r in res:
+        st = {"k": r["k"], "unitarity_K": _unitarity(r["K"]), "closed_l": int(r["diag"][10]),
+              "K_fit_asymmetry": float(np.abs(r["K_fit"] - r["K_fit"].T).max() / np.abs(r["K_fit"]).max()),
+              "K_fit_vs_K": float(np.abs(r["K_fit"] - r["K"]).max() / np.abs(r["K"]).max())}
+        print("  h2o_l20 " + " ".join(f"{k}={v:.3g}" for k, v in st.items()))
+        stats.append(st)
+    for st in stats:
+        assert st["unitarity_K"] < 1e-10
+    closed, open_ = stats
+    assert closed["closed_l"] > 0 and open_["closed_l"] == 0
+    # the reference's fit leaves the same order of asymmetry (the oracle's
+    # own |K_fit - K_fit^T| at these energies is ~1e-4 of |K_fit|)
+    assert open_["K_fit_asymmetry"] < 1e-3
+    # the two extractions differ by what the free-wave fit neglects of the
+    # potential's tail over the window (2e-3 on the CPU pin's homonuclear
+    # case; the polar molecule's 1/r^2 tail adds to it)
+    assert open_["K_fit_vs_K"] < 5e-2
+
+
+@pytest.mark.timeout(900)
+def test_full_size_delay_l20_pol(ctx):
+    """The analytic Wigner delay tau = Im sum d* dd/dE / sum |d|^2 (length
+    gauge) at the adjoint configuration's size against the FD-in-k delay on
+    two k spacings, extrapolated (the O(dk^2) truncation removed)."""
+    import torch
+    from sehf import observables as ob
+    from sehf import synth
+    p, ks_all = synth.config("h2o_l20_pol")
+    k0 = float(ks_all[31])
+    dk = 0.004
+    ks = np.array([k0, k0 - dk, k0 + dk, k0 - dk / 4, k0 + dk / 4])
+    res = _run(ctx, p, ks, tangent=(1.0, 0.0))
+    D = np.stack([r["D"] for r in res])
+    dD = np.stack([r["dD"] for r in res])
+    tau = float(ob.time_delay_analytic(torch.as_tensor(D[:1, 0:3].reshape(1, -1)),
+                                       torch.as_tensor(dD[:1, 0:3].reshape(1, -1)))[0])
+
+    def fd(i_lo, i_hi):
+        kk = torch.as_tensor(np.array([ks[i_lo], ks[0], ks[i_hi]]))
+        Dk = torch.as_tensor(np.stack([D[i_lo], D[0], D[i_hi]])[:, 0:3].reshape(3, -1))
+        return float(ob.time_delay(kk, Dk)[1])
+
+    t1, t2 = fd(1, 2), fd(3, 4)
+    t_conv = (16 * t2 - t1) / 15
+    print(f"  h2o_l20_pol k={k0:.4f}: tau analytic {tau:.9f}, fd(dk) {t1:.9f}, fd(dk/4) {t2:.9f}, "
+          f"extrapolated {t_conv:.9f}")
+    assert abs(tau - t_conv) < 1e-5 * max(1.0, abs(t_conv))

@@ -25,7 +25,8 @@ finite differences and by the analytic-vs-FD Wigner delay
   ~1e-4 of the identity, so max|dA| is ~1e-3 of max|dB| and a per-matrix
   measure of dA would weigh the rounding of dB-sized terms against a
   near-zero scale;
-* the forward outputs of a tangent run are bitwise those of a plain run;
+* the forward outputs of a tangent run are bitwise those of a plain run with
+  back-propagated dipoles (tangent runs always back-propagate);
 * a case with the exchange cut-off inside the grid (n_transition < Nr), where
   W^{-1} switches to block-diagonal form, and the plain CPU-runnable
   configuration (l_cont = 8, 3001 points) along E.
@@ -94,6 +95,9 @@ def test_tangent_directions_pol(ctx):
     dirs = np.array([[1.0, 0.0], [1.0, 0.0], [1.0, 0.0], [0.0, 1.0], [0.5, -2.0]])
     S = scan.Scan(p, ctx)
     got = S.run(ks, tangent=dirs)
+    # tangent runs back-propagate the dipoles: their forward outputs are those
+    # of a plain back-propagated run
+    S.set_dipole_path(scan.Scan.DIPOLES_BACKWARD)
     plain = S.run(ks)
     for e, (k, d) in enumerate(zip(ks, dirs)):
         ref, bound = _oracle_with_floor(O, p, float(k), tuple(d))

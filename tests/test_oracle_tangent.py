@@ -6,8 +6,9 @@ oracle's own forward outputs.
 * Richardson-extrapolated central differences (steps h and h/2) of K, A, B,
   d_raw, b_overlap and the reduced dipole D agree with the tangent to
   max(3e-7, 3 x the tangent's spread between two rounding orders) relative;
-  at k = 0.7 that is below 5e-7 (the extrapolated differences sit at ~1e-7), at
-  k = 1.1 the case's conditioning puts dD's rounding floor near 1e-6.  The smallest terms
+  at k = 0.7 that is below 1e-6 (the extrapolated differences sit at ~1e-7,
+  and the mutations below move the outputs by 2e-6 or more), at k = 1.1 the
+  case's conditioning puts dD's rounding floor near 1e-6.  The smallest terms
   of the tangent are O(h^2/6) ~ 7e-5 relative: dropping the boundary source
   h^2/6 w_N psi_N moves A, B, d_raw and b_overlap by ~5e-6 in the E case
   (D, normalised by the fit, absorbs it), and a weight taken one radial
@@ -76,7 +77,7 @@ def test_energy_tangent_matches_finite_differences(O, k):
     E = 0.5 * k * k
     o, tol = _tolerances(O, p, k, (1.0, 0.0))
     if k == 0.7:
-        assert max(tol.values()) < 5e-7           # the tight case keeps the mutation sensitivity
+        assert max(tol.values()) < 1e-6           # the tight case keeps the mutation sensitivity
     fd = _richardson(lambda e: O.solve(p, np.sqrt(2 * e)), E, 2e-4)
     for key, dkey in KEYS:
         an = o[dkey]
@@ -88,7 +89,7 @@ def test_alpha_tangent_matches_finite_differences(O):
     p = _problem()
     k = 0.7
     o, tol = _tolerances(O, p, k, (0.0, 1.0))
-    assert max(tol.values()) < 5e-7
+    assert max(tol.values()) < 1e-6
     fd = _richardson(lambda a: O.solve(dataclasses.replace(p, pol_alpha=a), k), 9.8, 0.04)
     for key, dkey in KEYS:
         an = o[dkey]
