@@ -265,6 +265,15 @@ int  sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, 
  * lanes accumulate, not the order of the K sum, so results are the same for
  * every policy. */
 int  sehf_set_gemm_tiles(int policy);
+/* Process-wide pivot order of the Gauss-Jordan inverses (the ratio matrices,
+ * the K match, the fit): 0 (default) tournament pivoting over blocks of 256
+ * rows (the panel step of communication-avoiding LU); 1 = partial pivoting,
+ * the pivot rows LU with partial pivoting chooses (LapackInverse.hpp:53-92
+ * uses Eigen's PartialPivLU), for orders up to 2048 (larger orders keep the
+ * tournament), one workgroup per matrix and about 1.3x the inverse time at
+ * order 1566.  $SEHF_GJ_LEAF=partial sets 1 at load.  Replaces no reference
+ * interface. */
+int  sehf_set_gj_pivoting(int mode);
 /* t[8]: cumulative seconds of setup, forward sweep, K matrix, back-propagation,
  * fit + dipole, per-batch host tables. */
 int  sehf_scan_stats(const sehf_scan* s, double* t);

@@ -58,6 +58,11 @@ struct GjWork {
 };
 void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int batch,
                 bool symmetrize, const GjWork& w, hipStream_t s);
+// Pivot order of gj_inverse (process-wide): 0 tournament pivoting, 1 partial
+// pivoting for n <= 2048 (one workgroup per matrix).  Initialised from
+// $SEHF_GJ_LEAF ("partial" selects 1).
+void set_gj_pivoting(int mode);
+int gj_pivoting();
 size_t gj_tbuf_elems(int n);   // doubles per matrix needed in tbuf
 size_t gj_nom_elems(int n);    // ints per matrix: leaf nominees
 size_t gj_stg_elems();         // doubles per matrix: leaf staging

@@ -25,6 +25,7 @@
 #include <mutex>
 #include <cstdlib>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace sehf {
@@ -885,6 +886,371 @@ __global__ __launch_bounds__(256) void k_gj_final(const double* mat, double* out
     }
 }
 
+// ---------------------------------------------------------------------------
+// Gauss-Jordan leaf in one workgroup per matrix, with partial pivoting
+// (orders up to WG_THREADS * R).  Thread t owns rows t + q * WG_THREADS, q < R,
+// and keeps their panel values in registers, so the pivot search of each
+// panel column runs over the whole matrix with one barrier per column:
+//
+//  phase 0  load the panel rows (and, with pre_w > 0, apply the left
+//           sibling's pivot block: P = [r in K_L ? 0 : P] + X(:, K_L) T);
+//           the pre-updated panel is written back for the staging reads.
+//  phase 1  for each panel column jj: every wave finds its largest |value|
+//           among its rows r >= c not yet chosen (lowest row on ties) and
+//           publishes it with that row's values; after the barrier every
+//           thread picks the overall largest (lowest row on ties), which is
+//           the row LU with partial pivoting chooses, and eliminates column
+//           jj from its candidate rows (values used for the selection only).
+//  phase 2  wave 0 turns the choice into the LAPACK interchange record and
+//           the move list, inverts the pivot block A_P and stages T (chosen
+//           rows x side range, for the right sibling) and the moved rows'
+//           old values in LDS, as the tournament merge does.
+//  phase 3  rows: X(P,K) = A_P^-1, X(O,K) = -A_O A_P^-1, and with post the
+//           side range update, exactly as k_leaf_apply.
+//  phase 4  the other columns receive the row moves (one column per thread).
+//
+// The arithmetic of the pre-update, the pivot block inverse and the apply is
+// that of the tournament kernels, so where both choose the same pivot rows
+// the results are bitwise the same.  Selected by SEHF_GJ_LEAF=partial: with
+// one compute unit per matrix and a barrier per pivot column it is slower
+// than the tournament leaf (about 120 against 60 us per leaf at order 1566),
+// and serves as the reference pivot order in the parity tests.
+// ---------------------------------------------------------------------------
+constexpr int WG_THREADS = 1024;
+constexpr int WG_WAVES = WG_THREADS / 64;
+
+__device__ __forceinline__ int wave_min_int(int x) {
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
+    x = min(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));
+    return min(min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+               min(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+}
+
+template <int R>
+__global__ __launch_bounds__(WG_THREADS) void k_leaf_wg(double* mat, int64_t ld, int64_t stride, int n, int c,
+                                                        int w, int pre_c, int pre_w, int xa0, int xw, int post,
+                                                        double* stg, int* piv, int* info) {
+    const int64_t b = blockIdx.x;
+    double* __restrict__ M = mat + b * stride;
+    double* S = stg + b * STG_D;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    // Row panels move between HBM and registers 16 lanes per row segment
+    // (coalesced) and are turned into one row per thread through this tile.
+    __shared__ double s_tile[WG_THREADS][GJW + 1];
+    __shared__ double s_T[GJW][GJW];                 // pre-update block, then T / T2
+    __shared__ double s_cv[2][WG_WAVES];             // per-wave best |value| (step parity)
+    __shared__ int s_cr[2][WG_WAVES];                // per-wave best row
+    __shared__ double s_crow[2][WG_WAVES][GJW];      // per-wave best row's values
+    __shared__ int s_chosen[GJW];
+    __shared__ double s_inv[GJW][GJW];
+    __shared__ double s_mv[2][2 * GJW][GJW];         // moved rows' old panel / side-range values
+    __shared__ int s_dst[2 * GJW], s_src[2 * GJW], s_cnt;
+    const int tj = t % GJW, tr = t / GJW;
+    constexpr int TRS = WG_THREADS / GJW;            // tile rows per pass
+    // coalesced load / store of rows [r0, r0 + WG_THREADS) x columns [col0, col0 + nc)
+    auto tile_load = [&](int r0, int col0, int nc, double (&g)[GJW]) {
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) {
+            const int rr = r0 + tr + it * TRS;
+            g[it] = (rr < n && tj < nc) ? M[(int64_t)rr * ld + col0 + tj] : 0.0;
+        }
+    };
+    auto tile_store = [&](int r0, int col0, int nc, const double (&g)[GJW]) {
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) {
+            const int rr = r0 + tr + it * TRS;
+            if (rr < n && tj < nc) M[(int64_t)rr * ld + col0 + tj] = g[it];
+        }
+    };
+    // registers (coalesced layout) -> own row, and back; the caller's barriers
+    // separate consecutive uses of the tile
+    auto to_row = [&](const double (&g)[GJW], double (&row)[GJW]) {
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) s_tile[tr + it * TRS][tj] = g[it];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) row[j] = s_tile[t][j];
+        __syncthreads();
+    };
+    auto from_row = [&](const double (&row)[GJW], double (&g)[GJW]) {
+#pragma unroll
+        for (int j = 0; j < GJW; ++j) s_tile[t][j] = row[j];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < GJW; ++it) g[it] = s_tile[tr + it * TRS][tj];
+        __syncthreads();
+    };
+
+    // ---- phase 0: panel rows (+ pre-update, written back) ----
+    double v[R][GJW];
+    if (pre_w > 0 && t < GJW * GJW) s_T[t / GJW][t % GJW] = stg_T(S)[t];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r0 = q * WG_THREADS, r = r0 + t;
+        double g[GJW];
+        tile_load(r0, c, w, g);
+        to_row(g, v[q]);
+        if (pre_w > 0) {
+            tile_load(r0, pre_c, pre_w, g);
+            double xv[GJW];
+            to_row(g, xv);
+            const bool inK = (r >= pre_c && r < pre_c + pre_w);
+            double acc[GJW];
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) acc[j] = inK ? 0.0 : v[q][j];
+#pragma unroll
+            for (int kk = 0; kk < GJW; ++kk) {
+                if (kk < pre_w) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) acc[j] = fma(xv[kk], s_T[kk][j], acc[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) v[q][j] = (r < n && j < w) ? acc[j] : 0.0;
+            from_row(v[q], g);
+            tile_store(r0, c, w, g);
+        }
+    }
+    // ---- phase 1: partial pivoting over all rows r >= c ----
+    unsigned int csel = 0;
+    // unrolled over the 16 columns so that every register index is static
+#pragma unroll
+    for (int jj = 0; jj < GJW; ++jj) {
+        if (jj < w) {
+            const int par = jj & 1;
+            double mine = -1.0;
+            int qb = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int r = t + q * WG_THREADS;
+                const double a = fabs(v[q][jj]);
+                if (r >= c && r < n && !((csel >> q) & 1u) && a > mine) { mine = a; qb = q; }
+            }
+            const double wm = wave_max(mine);
+            const int my_row = (wm > 0.0 && mine == wm) ? t + qb * WG_THREADS : INT_MAX;
+            const int wr = wave_min_int(my_row);
+            if (wr == INT_MAX) {
+                if (lane == 0) { s_cv[par][wv] = -1.0; s_cr[par][wv] = INT_MAX; }
+            } else if (my_row == wr) {
+                s_cv[par][wv] = wm;
+                s_cr[par][wv] = wr;
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) {
+                    double x = v[0][j];
+#pragma unroll
+                    for (int q = 1; q < R; ++q) x = (qb == q) ? v[q][j] : x;
+                    s_crow[par][wv][j] = x;
+                }
+            }
+            __syncthreads();
+            double best = -1.0;
+            int brow = INT_MAX, bw = 0;
+#pragma unroll
+            for (int x = 0; x < WG_WAVES; ++x) {
+                const double cvx = s_cv[par][x];
+                const int crx = s_cr[par][x];
+                if (cvx > best || (cvx == best && crx < brow)) { best = cvx; brow = crx; bw = x; }
+            }
+            if (t == 0) s_chosen[jj] = (best > 0.0) ? brow : -1;
+            if (best > 0.0) {
+                double prow[GJW];
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) prow[j] = (j >= jj) ? s_crow[par][bw][j] : 0.0;
+                const double pinv = 1.0 / prow[jj];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int r = t + q * WG_THREADS;
+                    if (r == brow) { csel |= 1u << q; continue; }
+                    if (r >= c && r < n && !((csel >> q) & 1u)) {
+                        const double f = v[q][jj] * pinv;
+#pragma unroll
+                        for (int j = 0; j < GJW; ++j)
+                            if (j > jj) v[q][j] = fma(-f, prow[j], v[q][j]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- phase 2: interchange record, pivot block inverse, staging (wave 0) ----
+    if (wv == 0) {
+        int chosen = (lane < w) ? s_chosen[lane] : -1;
+        // rank-deficient panel: fill empty steps with rows of [c, c+w) not yet
+        // chosen (one exists for every empty step) and flag the matrix
+        int sing = 0;
+        for (int jj = 0; jj < w; ++jj) {
+            if (__builtin_amdgcn_readlane(chosen, jj) >= 0) continue;
+            if (sing == 0) sing = c + jj + 1;
+            bool fr = lane < w;
+            for (int j2 = 0; j2 < w; ++j2)
+                if (__builtin_amdgcn_readlane(chosen, j2) == c + lane) fr = false;
+            const unsigned long long fb = __ballot(fr);
+            const int rowp = c + __ffsll(static_cast<long long>(fb)) - 1;
+            if (lane == jj) chosen = rowp;
+        }
+        // interchange record: slot s < w is position c+s, slot w+i is position
+        // chosen_i when it lies below the panel rows; content = original row
+        // now at the slot's position
+        const int prs = __shfl(chosen, (lane >= w && lane < 2 * w) ? lane - w : 0);
+        int pos = -1;
+        if (lane < w) pos = c + lane;
+        else if (lane < 2 * w) pos = (prs >= c + w) ? prs : -1;
+        int content = pos;
+        int mypiv = 0;
+        for (int jj = 0; jj < w; ++jj) {
+            const int pr = __builtin_amdgcn_readlane(chosen, jj);
+            const unsigned long long hb = __ballot(pos >= 0 && content == pr);
+            const int bs = __ffsll(static_cast<long long>(hb)) - 1;
+            const int pp = __builtin_amdgcn_readlane(pos, bs);
+            const int va = __builtin_amdgcn_readlane(content, jj);
+            const int vb = __builtin_amdgcn_readlane(content, bs);
+            if (lane == jj) { content = vb; mypiv = pp; }
+            else if (lane == bs) content = va;
+        }
+        const bool mv = pos >= 0 && content != pos;
+        const unsigned long long bal = __ballot(mv);
+        // T (chosen rows x side range) and the moved rows' old values go
+        // straight to LDS (and T to the sibling's staging)
+        {
+            const int pr = (lane < w) ? chosen : -1;
+            if (lane < GJW && xw > 0) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) {
+                    const double tv = (pr >= 0 && j < xw) ? M[(int64_t)pr * ld + xa0 + j] : 0.0;
+                    s_T[lane][j] = tv;
+                    stg_T(S)[lane * GJW + j] = tv;
+                }
+            }
+            if (mv) {
+                const int at = __popcll(bal & ((1ull << lane) - 1ull));
+                s_dst[at] = pos;
+                s_src[at] = content;
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) {
+                    s_mv[0][at][j] = (j < w) ? M[(int64_t)content * ld + c + j] : 0.0;
+                    s_mv[1][at][j] = (j < xw) ? M[(int64_t)content * ld + xa0 + j] : 0.0;
+                }
+            }
+        }
+        double a[GJW];
+        {
+            const int pr = (lane < w) ? chosen : -1;
+#pragma unroll
+            for (int j = 0; j < GJW; ++j)
+                a[j] = (pr >= 0 && j < w) ? M[(int64_t)pr * ld + c + j] : ((lane == j) ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < GJW; ++k) {
+            double pk[GJW];
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) pk[j] = lane_read(a[j], k);
+            const double pinv = 1.0 / pk[k];
+            if (lane == k) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? pinv : pk[j] * pinv;
+            } else {
+                const double f = a[k] * pinv;
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) a[j] = (j == k) ? -f : fma(-f, pk[j], a[j]);
+            }
+        }
+        if (lane < GJW) {
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) s_inv[lane][j] = a[j];
+        }
+        if (lane == 0) {
+            s_cnt = __popcll(bal);
+            if (sing != 0 && info[b] == 0) info[b] = sing;
+        }
+        if (lane < w) piv[b * n + c + lane] = mypiv;
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    // ---- phase 3: panel rows and side range ----
+#pragma unroll 1
+    for (int q = 0; q < R; ++q) {
+        asm volatile("" ::: "memory");   // keeps the LDS operands from being hoisted out of the loop
+        const int r0 = q * WG_THREADS, r = r0 + t;
+        int e_src = -1;
+        for (int e = 0; e < cnt; ++e)
+            if (s_dst[e] == r) e_src = e;
+        const bool inK = (r >= c && r < c + w);
+        double g[GJW], o[GJW];
+        {
+            double a[GJW];
+            tile_load(r0, c, w, g);
+            to_row(g, a);
+            if (e_src >= 0) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) a[j] = s_mv[0][e_src][j];
+            }
+#pragma unroll
+            for (int j = 0; j < GJW; ++j) o[j] = 0.0;
+            if (inK) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) o[j] = s_inv[r - c][j];
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < GJW; ++kk) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) o[j] = fma(-a[kk], s_inv[kk][j], o[j]);
+                }
+            }
+        }
+        from_row(o, g);
+        tile_store(r0, c, w, g);
+        if (post) {
+            double x[GJW];
+            tile_load(r0, xa0, xw, g);
+            to_row(g, x);
+            if (inK) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) x[j] = 0.0;
+            } else if (e_src >= 0) {
+#pragma unroll
+                for (int j = 0; j < GJW; ++j) x[j] = s_mv[1][e_src][j];
+            }
+#pragma unroll
+            for (int kk = 0; kk < GJW; ++kk) {
+                if (kk < w) {
+#pragma unroll
+                    for (int j = 0; j < GJW; ++j) x[j] = fma(o[kk], s_T[kk][j], x[j]);
+                }
+            }
+            from_row(x, g);
+            tile_store(r0, xa0, xw, g);
+        }
+    }
+    // ---- phase 4: row moves in the other columns ----
+    if (cnt > 0) {
+        for (int jc = t; jc < n; jc += WG_THREADS) {
+            if ((jc >= c && jc < c + w) || (post && jc >= xa0 && jc < xa0 + xw)) continue;
+            // all sources of this column are read before any destination is
+            // written (the two sets overlap); 2w <= 32 values per column
+            double xm[2 * GJW];
+#pragma unroll
+            for (int e = 0; e < 2 * GJW; ++e) xm[e] = (e < cnt) ? M[(int64_t)s_src[e] * ld + jc] : 0.0;
+#pragma unroll
+            for (int e = 0; e < 2 * GJW; ++e)
+                if (e < cnt) M[(int64_t)s_dst[e] * ld + jc] = xm[e];
+        }
+    }
+}
+
+namespace {
+std::atomic<int> g_gj_pivoting{[] {
+    const char* e = std::getenv("SEHF_GJ_LEAF");
+    return (e && std::string(e) == "partial") ? 1 : 0;
+}()};
+}  // namespace
+void set_gj_pivoting(int mode) {
+    if (mode < 0 || mode > 1) throw std::runtime_error("Gauss-Jordan pivoting mode must be 0 or 1");
+    g_gj_pivoting.store(mode);
+}
+int gj_pivoting() { return g_gj_pivoting.load(); }
+
 size_t gj_tbuf_elems(int n) { return static_cast<size_t>(n) * n; }
 
 size_t gj_nom_elems(int n) { return static_cast<size_t>((n + LEAF_BLK - 1) / LEAF_BLK) * GJW; }
@@ -901,6 +1267,17 @@ static void gj_leaf(double* mat, int n, int64_t ld, int64_t stride, int batch, i
     if (pre_w > 0) fl += 2.0 * n * static_cast<double>(pre_w) * w;
     if (post) fl += 2.0 * n * static_cast<double>(w) * xw;
     ProfScope ps(s, fl * batch, 1);
+    // Partial pivoting (orders up to 2 * WG_THREADS): the one-workgroup leaf,
+    // 1.3x the inverse time of the tournament leaf at order 1566, batch 48.
+    if (g_gj_pivoting.load(std::memory_order_relaxed) == 1 && n <= 2 * WG_THREADS) {
+        if (n <= WG_THREADS)
+            hipLaunchKernelGGL(k_leaf_wg<1>, dim3(batch), dim3(WG_THREADS), 0, s, mat, ld, stride, n, c, w, pre_c,
+                               pre_w, xa0, xw, post, W.stg, W.piv, W.info);
+        else
+            hipLaunchKernelGGL(k_leaf_wg<2>, dim3(batch), dim3(WG_THREADS), 0, s, mat, ld, stride, n, c, w, pre_c,
+                               pre_w, xa0, xw, post, W.stg, W.piv, W.info);
+        return;
+    }
     const int nblk = (n + LEAF_BLK - 1) / LEAF_BLK;
     const int nc = (nblk * GJW + 63) / 64;
 #define SEHF_NOMINATE(NC)                                                                                   \

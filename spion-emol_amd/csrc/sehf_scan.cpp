@@ -1600,6 +1600,10 @@ int sehf_set_gemm_tiles(int policy) {
     return guarded([&] { set_gemm_tiles(policy); });
 }
 
+int sehf_set_gj_pivoting(int mode) {
+    return guarded([&] { set_gj_pivoting(mode); });
+}
+
 int sehf_scan_stats(const sehf_scan* s, double* t) {
     return guarded([&] {
         for (int i = 0; i < 8; ++i) {

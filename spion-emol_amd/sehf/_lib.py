@@ -57,6 +57,7 @@ SIGNATURES = [
     ("sehf_scan_autotune", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp,
                                           _ip]),
     ("sehf_set_gemm_tiles", ctypes.c_int, [ctypes.c_int]),
+    ("sehf_set_gj_pivoting", ctypes.c_int, [ctypes.c_int]),
     ("sehf_scan_destroy", None, [_vp]),
     ("sehf_scan_stats", ctypes.c_int, [_vp, _dp]),
     ("sehf_scan_set_groups", ctypes.c_int, [_vp, ctypes.c_int]),

@@ -41,3 +41,35 @@ def test_gj_orders(ctx, n):
     ref = 0.5 * (ref + ref.T)
     print(n, rel_err(got, ref))
     assert rel_err(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 5, 16, 17, 33, 100, 1024, 1025, 1566])
+def test_gj_orders_partial_pivoting(ctx, n):
+    """The one-workgroup leaf with partial pivoting (sehf_set_gj_pivoting(1)),
+    one and two rows per thread, against numpy's inverse; the tournament
+    leaf's result on the same matrix agrees to rounding."""
+    from sehf._lib import check, load
+    from sehf.steppers import GpuSinvStepper
+    L = load()
+    rng = np.random.default_rng(7 * n + 1)
+    A = rng.standard_normal((n, n))
+    st = GpuSinvStepper(ctx, n, 0)
+    check(L.sehf_set_gj_pivoting(1), "sehf_set_gj_pivoting")
+    try:
+        got, _ = st.step_inverse_only(A)
+    finally:
+        check(L.sehf_set_gj_pivoting(0), "sehf_set_gj_pivoting")
+    tour, _ = st.step_inverse_only(A)
+    ref = np.linalg.inv(A)
+    ref = 0.5 * (ref + ref.T)
+    print(n, rel_err(got, ref), rel_err(got, tour))
+    assert rel_err(got, ref) < 1e-10
+    assert rel_err(got, tour) < 1e-10
+
+
+def test_gj_pivoting_mode_rejects_bad_values(ctx):
+    from sehf._lib import load
+    L = load()
+    assert L.sehf_set_gj_pivoting(2) != 0
+    assert L.sehf_set_gj_pivoting(-1) != 0
+    assert L.sehf_set_gj_pivoting(0) == 0

@@ -96,6 +96,32 @@ def test_scan_h2_l8_full(ctx):
     _compare(p, ks, got, ref, label="h2_l8", ext_list=ext, strict=True)
 
 
+def test_scan_h2_l8_full_partial_pivoting(ctx):
+    """The same configuration with every Gauss-Jordan inverse pivoted as LU
+    with partial pivoting (the reference's pivot order, sehf_set_gj_pivoting(1))
+    instead of tournament pivoting: the same flat bounds hold."""
+    import oracle_py as O
+    from sehf import scan, synth
+    from sehf._lib import check, load
+    L = load()
+    p, ks = synth.config("h2_l8")
+    check(L.sehf_set_gj_pivoting(1), "sehf_set_gj_pivoting")
+    try:
+        got = scan.solve_scan(p, ks, ctx=ctx)
+    finally:
+        check(L.sehf_set_gj_pivoting(0), "sehf_set_gj_pivoting")
+    ref = O.solve_scan(p, ks)
+    ext = O.solve_scan_rounded_inverses(p, ks)
+    _compare(p, ks, got, ref, label="h2_l8 partial pivoting", ext_list=ext, strict=True)
+    # On this configuration the tournament picks the partial-pivoting rows at
+    # every leaf of every inverse, and the two leaves share their arithmetic,
+    # so the default run is bitwise the same (the kernels are deterministic).
+    tour = scan.solve_scan(p, ks, ctx=ctx)
+    for a, b in zip(got, tour):
+        for key in KEYS:
+            assert np.array_equal(a[key], b[key]), (key, a["k"])
+
+
 def test_scan_groups_bitwise(ctx):
     """Energies are independent: batching and concurrent groups (own streams,
     own host threads) must not change a single bit of any output."""
