@@ -310,6 +310,75 @@ int  sehf_scan_set_groups(sehf_scan* s, int groups);
 int  sehf_scan_set_dipole_path(sehf_scan* s, int path);
 void sehf_scan_destroy(sehf_scan* s);
 
+/* ---- device storage plan ------------------------------------------------
+ * The device counterpart of the reference's StoragePlanner
+ * (src/scatt/StoragePlanner.hpp:106-117, plan_storage), which places the
+ * four big per-radial-point objects (pot, sinv, rinv, psi) in RAM or in
+ * chunked disk files under a memory budget.  On the device:
+ *   pot   the energy-independent S0_n = I - h^2/6 V_n - B_n D_n^{-1} B_n^T
+ *         and B_n D_n^{-1}: SEHF_TABLES_RESIDENT keeps them for every n in
+ *         HBM (N_psi^2 Nr + N_psi N_f n_transition doubles);
+ *         SEHF_TABLES_STREAMED keeps only their compact inputs and rebuilds
+ *         them per radial step on the device, on the side stream, with the
+ *         resident build's kernels (bitwise the same values);
+ *   sinv, rinv  are never stored for every n: ratio matrices are kept at
+ *         checkpoints every seg_len points and recomputed per segment
+ *         (seg_len x (N_total^2 + N_psi^2) doubles per energy of segment
+ *         buffers); the checkpoints live in HBM (SEHF_CKPT_DEVICE) or in
+ *         pinned host memory (SEHF_CKPT_HOST), the reference's DISK level;
+ *   psi   is never stored (forward-accumulated dipole rows, or integrands
+ *         formed during back-propagation).
+ * sehf_plan_storage chooses tables, checkpoint level, segment length and
+ * batch width: greedily, tables resident while that does not cost batch
+ * width, checkpoints on the device while a batch fits, the segment length
+ * that minimises bytes per energy when the default one leaves no batch,
+ * and a batch of at most min(n_energies, batch_cap) energies (batch_cap <= 0:
+ * 64, where the l_cont = 30 scan's throughput stops rising on MI355X).
+ * device_bytes = 0 queries the current device's free memory.  Fails when
+ * not even one energy fits. */
+#define SEHF_TABLES_RESIDENT 0
+#define SEHF_TABLES_STREAMED 1
+#define SEHF_CKPT_DEVICE 0
+#define SEHF_CKPT_HOST 1
+typedef struct {
+    int    Nr;                /* radial points                                      */
+    int    n_psi, n_f;        /* channel blocks (N_total = n_psi + n_f)             */
+    int    n_transition;      /* exchange cut-off index                             */
+    int    n_occ;             /* occupied orbitals                                  */
+    int    n_exp, n_lambda;   /* table inputs per point: V_s terms, chi per orbital */
+    int    tangent;           /* derivative runs (sehf_scan_run_tangent)            */
+    int    dipole_path;       /* SEHF_DIPOLES_FORWARD / _BACKWARD                   */
+    int    n_energies;        /* energies to run                                    */
+    int    batch_cap;         /* <= 0: 64                                           */
+    size_t device_bytes;      /* HBM available; 0 = query the device                */
+    size_t host_bytes;        /* pinned host memory for checkpoints; 0 = none       */
+    size_t runtime_bytes;     /* kept for the runtime and libraries                 */
+    double reserve_fraction;  /* of what remains, kept free (the reference: 0.10)   */
+} sehf_plan_in;
+typedef struct {
+    int    tables;            /* SEHF_TABLES_*                                      */
+    int    ckpt;              /* SEHF_CKPT_*                                        */
+    int    seg_len;           /* checkpoint spacing L                               */
+    int    n_ckpt;            /* checkpoints per energy                             */
+    int    batch;             /* energies per batch                                 */
+    size_t budget_bytes;      /* (device - runtime) x (1 - reserve)                 */
+    size_t shared_bytes;      /* energy-independent device bytes                    */
+    size_t per_energy_bytes;  /* device bytes per energy of a batch                 */
+    size_t host_per_energy_bytes; /* pinned host bytes per energy (host checkpoints) */
+    char   report[1536];      /* multi-line summary                                 */
+} sehf_plan;
+int  sehf_plan_storage(const sehf_plan_in* in, sehf_plan* out);
+/* sehf_scan_create with the energy-independent tables resident or streamed
+ * (SEHF_TABLES_*; sehf_scan_create is RESIDENT, or STREAMED when
+ * $SEHF_TABLES=streamed). */
+int  sehf_scan_create_with(sehf_ctx* ctx, const sehf_problem* p, int tables, sehf_scan** out);
+/* Where the scan keeps its ratio-matrix checkpoints (SEHF_CKPT_*, default
+ * device); takes effect when the batch buffers are next allocated. */
+int  sehf_scan_set_checkpoints(sehf_scan* s, int where);
+/* Device bytes the handle holds: [0] energy-independent tables and inputs,
+ * [1] batch buffers of all groups, [2] pinned host bytes (checkpoints). */
+int  sehf_scan_memory(const sehf_scan* s, size_t* bytes);
+
 /* Kernel timing: between begin and end, every `every`-th FP64 GEMM and
  * Gauss-Jordan leaf launch is bracketed by HIP events on its own stream.
  * out[16] = five triples {ms, flop, launches} over the sampled launches:

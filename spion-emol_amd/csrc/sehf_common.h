@@ -20,29 +20,39 @@
 
 namespace sehf {
 
-// Owning device allocation.
+// Owning device allocation, or pinned host memory (alloc_host: page-locked,
+// mapped into the device's address space, so kernels and copies reach it).
 template <class T>
 struct DevArray {
     T* p = nullptr;
     size_t n = 0;
+    bool host = false;
     DevArray() = default;
     explicit DevArray(size_t count) { alloc(count); }
     DevArray(const DevArray&) = delete;
     DevArray& operator=(const DevArray&) = delete;
-    DevArray(DevArray&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevArray(DevArray&& o) noexcept : p(o.p), n(o.n), host(o.host) { o.p = nullptr; o.n = 0; }
     DevArray& operator=(DevArray&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) { release(); p = o.p; n = o.n; host = o.host; o.p = nullptr; o.n = 0; }
         return *this;
     }
     ~DevArray() { release(); }
     void alloc(size_t count) {
         release();
+        host = false;
         if (count == 0) return;
         SEHF_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
         n = count;
     }
+    void alloc_host(size_t count) {
+        release();
+        host = true;
+        if (count == 0) return;
+        SEHF_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)(host ? hipHostFree(p) : hipFree(p));
         p = nullptr; n = 0;
     }
     size_t bytes() const { return n * sizeof(T); }

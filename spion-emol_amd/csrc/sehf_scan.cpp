@@ -32,6 +32,7 @@
 #include "../../include/sehf.h"
 #include "sehf_common.h"
 #include "sehf_host_math.h"
+#include "sehf_plan.h"
 #include "sehf_scan_kernels.h"
 #include "sehf_setup.h"
 #include "sehf_setup_kernels.h"
@@ -51,6 +52,10 @@ struct Batch {
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
     DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
     DevArray<double> Ad, Ad2;   // forward-accumulated dipole rows (ns x Nt per energy, ping-pong)
+    // streamed tables (energy independent, shared by the batch): S0_n, V_n and
+    // B_n scratch of the side stream, B D^{-1} of the two forward ring slots
+    // and of every point of the segment being recomputed
+    DevArray<double> tS0, tV, tB, ringBD, segBD;
     GjBuffers gj;             // inversions on the main stream
     GjBuffers gj2;            // Schur inversions on the side stream
     // tangent along (E, alpha), allocated by the first tangent run
@@ -102,6 +107,18 @@ struct sehf_scan {
     int Np = 0, Nf = 0, Nt = 0, Nr = 0, n_tr = 0, n_start = 1, n_pts = 0, nslot = 0, n_bd = 0, nstab = 0;
     double h = 0, c6 = 0;
     DevArray<double> S0, BD, Dinv, VN, BN, negDinvN, Drows, dcorr, cvd, bdb;
+    // Streamed tables (SEHF_TABLES_STREAMED): S0_n and B_n D_n^{-1} are not
+    // kept for every n but rebuilt per radial step on the device from these
+    // inputs (V_s(r_n), chi(r_n), the per-point factors, the Gaunt tables),
+    // by the kernels and in the order of the resident build, so bitwise equal.
+    int tables = SEHF_TABLES_RESIDENT;
+    struct TableInputs {
+        DevArray<int> pi, pj, vsig, lmu, xlam;
+        DevArray<int64_t> vptr, xptr;
+        DevArray<double> vval, xval, vs, vp, ir2, chi, fac;
+        DevArray<unsigned char> live;
+        int64_t npairs = 0;
+    } tin;
     std::vector<double> dcorr_h, vdiag;       // vdiag: (n_start+2) x Np potential diagonal
     std::vector<double> gA, gS, lA, lS;
     std::vector<char> hasLA, hasLS;
@@ -111,6 +128,7 @@ struct sehf_scan {
     int ctx_device = 0;
     int dipole_path = SEHF_DIPOLES_FORWARD;    // how the dipole integrals are summed (sehf_scan_set_dipole_path)
     bool sinv_spd = true;                       // Sinv by SPD block elimination (else pivoted Gauss-Jordan)
+    int ckpt_where = SEHF_CKPT_DEVICE;          // ratio-matrix checkpoints in HBM or pinned host memory
     std::mutex host_mu;   // lazily computed host tables (stability eigenvalues)
     size_t dev_bytes_shared = 0;
     double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
@@ -141,13 +159,16 @@ void build_shared(sehf_scan* s) {
     hipStream_t st = s->st;
     const int Np = s->Np, Nf = s->Nf, Nr = s->Nr;
     const size_t pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * Nf;
-    s->S0.alloc(pp * Nr);
+    const bool streamed = s->tables == SEHF_TABLES_STREAMED;
     s->cvd.alloc(static_cast<size_t>(Np) * Nr);
     s->bdb.alloc(static_cast<size_t>(Np) * Nr);
     fill(s->bdb.p, static_cast<size_t>(Np) * Nr, 0.0, st);
     std::vector<double> cvd_h(static_cast<size_t>(Np) * Nr);
     s->n_bd = std::min(s->n_tr, Nr);
-    if (Nf > 0) s->BD.alloc(pf * s->n_bd);
+    if (!streamed) {
+        s->S0.alloc(pp * Nr);
+        if (Nf > 0) s->BD.alloc(pf * s->n_bd);
+    }
     s->Dinv.alloc(static_cast<size_t>(std::max(Nf, 1)) * Nr);
     {
         std::vector<double> dh(static_cast<size_t>(std::max(Nf, 1)) * Nr, 0.0);
@@ -162,34 +183,49 @@ void build_shared(sehf_scan* s) {
     // and the sparse Gaunt tables (sehf_setup_kernels.hip); with
     // SEHF_TABLES_ON_HOST set they are built on the host and uploaded instead
     // (the reference path of the equivalence test: the tables are bitwise
-    // equal).
-    const bool on_host = std::getenv("SEHF_TABLES_ON_HOST") != nullptr;
+    // equal).  The inputs of the device build are uploaded for every point at
+    // once; streamed tables keep them for the per-step rebuild.
+    const bool on_host = std::getenv("SEHF_TABLES_ON_HOST") != nullptr && !streamed;
     const int chunk = 32;
     const int n_exp = S.n_exp, n_lam = S.n_lambda, n_occ = S.n_occ;
     std::vector<double> Vh(on_host ? pp * chunk : pp), Bh(on_host ? std::max<size_t>(pf, 1) * chunk : 1);
     DevArray<double> Vd(pp * chunk), Bd(std::max<size_t>(pf, 1) * chunk);
-    DevArray<int> d_pi, d_pj, d_vsig, d_lmu, d_xlam;
-    DevArray<int64_t> d_vptr, d_xptr;
-    DevArray<double> d_vval, d_xval, d_vs, d_vp, d_ir2, d_chi, d_fac;
-    DevArray<unsigned char> d_live;
-    std::vector<double> vs_h, vp_h(chunk), ir2_h(chunk), chi_h, fac_h(chunk);
-    std::vector<unsigned char> live_h(chunk);
+    // chunk-local S0 and B D^{-1} when they are not kept for every point
+    DevArray<double> S0k, BDk;
+    if (streamed) { S0k.alloc(pp * chunk); BDk.alloc(std::max<size_t>(pf, 1) * chunk); }
+    auto& T = s->tin;
     auto upi = [&](auto& d, const auto& h) {
         d.alloc(std::max<size_t>(h.size(), 1));
         if (!h.empty())
             SEHF_HIP_CHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, st));
+        SEHF_HIP_CHECK(hipStreamSynchronize(st));
     };
     if (!on_host) {
-        upi(d_pi, S.pot_pair_i()); upi(d_pj, S.pot_pair_j()); upi(d_vptr, S.pot_ptr());
-        upi(d_vsig, S.pot_sigma()); upi(d_vval, S.pot_val()); upi(d_lmu, S.l_mu);
-        upi(d_xptr, S.exch_ptr()); upi(d_xlam, S.exch_lambda()); upi(d_xval, S.exch_val());
-        vs_h.resize(static_cast<size_t>(chunk) * n_exp);
-        chi_h.resize(static_cast<size_t>(chunk) * n_occ * n_lam);
-        d_vs.alloc(vs_h.size()); d_vp.alloc(chunk); d_ir2.alloc(chunk); d_live.alloc(chunk);
-        d_chi.alloc(std::max<size_t>(chi_h.size(), 1)); d_fac.alloc(chunk);
-        SEHF_HIP_CHECK(hipStreamSynchronize(st));
+        upi(T.pi, S.pot_pair_i()); upi(T.pj, S.pot_pair_j()); upi(T.vptr, S.pot_ptr());
+        upi(T.vsig, S.pot_sigma()); upi(T.vval, S.pot_val()); upi(T.lmu, S.l_mu);
+        upi(T.xptr, S.exch_ptr()); upi(T.xlam, S.exch_lambda()); upi(T.xval, S.exch_val());
+        T.npairs = static_cast<int64_t>(S.pot_pair_i().size());
+        std::vector<double> vp_h(Nr), ir2_h(Nr), fac_h(Nr);
+        std::vector<unsigned char> live_h(Nr);
+        for (int n = 0; n < Nr; ++n) {
+            const double r = n * s->h;
+            live_h[n] = (r > 1e-14) ? 1 : 0;
+            vp_h[n] = S.polarization(r);
+            ir2_h[n] = 1.0 / (r * r);
+            fac_h[n] = (r < 1e-10) ? 0.0 : 2.0 * std::sqrt(2.0 * M_PI) / r;
+        }
+        upi(T.vp, vp_h); upi(T.ir2, ir2_h); upi(T.fac, fac_h); upi(T.live, live_h);
+        {
+            std::vector<double> vs_h(static_cast<size_t>(Nr) * n_exp);
+            S.vsigma_table(0, Nr, vs_h.data());
+            upi(T.vs, vs_h);
+        }
+        if (Nf > 0 && s->n_bd > 0) {
+            std::vector<double> chi_h(static_cast<size_t>(s->n_bd) * n_occ * n_lam);
+            S.chi_table(0, s->n_bd, chi_h.data());
+            upi(T.chi, chi_h);
+        }
     }
-    const int64_t npairs = static_cast<int64_t>(S.pot_pair_i().size());
     const double h2_12 = s->h * s->h / 12.0;
     for (int n0 = 0; n0 < Nr; n0 += chunk) {
         const int nc = std::min(chunk, Nr - n0);
@@ -203,42 +239,26 @@ void build_shared(sehf_scan* s) {
             if (Nf > 0 && nb > 0)
                 SEHF_HIP_CHECK(hipMemcpyAsync(Bd.p, Bh.data(), nb * pf * sizeof(double), hipMemcpyHostToDevice, st));
         } else {
-            S.vsigma_table(n0, nc, vs_h.data());
-            for (int q = 0; q < nc; ++q) {
-                const double r = (n0 + q) * s->h;
-                live_h[q] = (r > 1e-14) ? 1 : 0;
-                vp_h[q] = S.polarization(r);
-                ir2_h[q] = 1.0 / (r * r);
-                fac_h[q] = (r < 1e-10) ? 0.0 : 2.0 * std::sqrt(2.0 * M_PI) / r;
-            }
-            SEHF_HIP_CHECK(hipMemcpyAsync(d_vs.p, vs_h.data(), static_cast<size_t>(nc) * n_exp * sizeof(double),
-                                          hipMemcpyHostToDevice, st));
-            SEHF_HIP_CHECK(hipMemcpyAsync(d_vp.p, vp_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
-            SEHF_HIP_CHECK(hipMemcpyAsync(d_ir2.p, ir2_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
-            SEHF_HIP_CHECK(hipMemcpyAsync(d_live.p, live_h.data(), nc, hipMemcpyHostToDevice, st));
-            SEHF_HIP_CHECK(hipMemcpyAsync(d_fac.p, fac_h.data(), nc * sizeof(double), hipMemcpyHostToDevice, st));
-            build_V(Vd.p, d_vs.p, n_exp, d_pi.p, d_pj.p, d_vptr.p, d_vsig.p, d_vval.p, npairs, Np, d_lmu.p, d_vp.p,
-                    d_ir2.p, d_live.p, nc, st);
+            build_V(Vd.p, T.vs.p + static_cast<size_t>(n0) * n_exp, n_exp, T.pi.p, T.pj.p, T.vptr.p, T.vsig.p,
+                    T.vval.p, T.npairs, Np, T.lmu.p, T.vp.p + n0, T.ir2.p + n0, T.live.p + n0, nc, st);
             scaled_diag(s->cvd.p + static_cast<size_t>(n0) * Np, Vd.p, s->c6, Np, nc, st);
-            if (Nf > 0 && nb > 0) {
-                S.chi_table(n0, nb, chi_h.data());
-                SEHF_HIP_CHECK(hipMemcpyAsync(d_chi.p, chi_h.data(), static_cast<size_t>(nb) * n_occ * n_lam * sizeof(double),
-                                              hipMemcpyHostToDevice, st));
-                build_B(Bd.p, d_chi.p, n_occ, n_lam, d_xptr.p, d_xlam.p, d_xval.p, Np, S.n_sigma, d_fac.p, h2_12, nb, st);
-            }
+            if (Nf > 0 && nb > 0)
+                build_B(Bd.p, T.chi.p + static_cast<size_t>(n0) * n_occ * n_lam, n_occ, n_lam, T.xptr.p, T.xlam.p,
+                        T.xval.p, Np, S.n_sigma, T.fac.p + n0, h2_12, nb, st);
         }
-        double* S0c = s->S0.p + static_cast<size_t>(n0) * pp;
+        double* S0c = streamed ? S0k.p : s->S0.p + static_cast<size_t>(n0) * pp;
+        double* BDc = streamed ? BDk.p : (Nf > 0 ? s->BD.p + static_cast<size_t>(n0) * pf : nullptr);
         // S0 = I - h^2/6 V
         axpbyi(S0c, Np, pp, Vd.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, nc, st);
         if (Nf > 0 && nb > 0) {
             for (int q = 0; q < nb; ++q)
-                col_scale(s->BD.p + static_cast<size_t>(n0 + q) * pf, Np, Bd.p + q * pf, Np,
-                          s->Dinv.p + static_cast<size_t>(n0 + q) * Nf, Np, Nf, st);
+                col_scale(BDc + q * pf, Np, Bd.p + q * pf, Np, s->Dinv.p + static_cast<size_t>(n0 + q) * Nf, Np, Nf,
+                          st);
             // (B D^{-1} B^T)_ii as a separate dot product for the per-energy diagonal
-            row_dot(s->bdb.p + static_cast<size_t>(n0) * Np, s->BD.p + static_cast<size_t>(n0) * pf, Bd.p, Np, Nf, nb, st);
+            row_dot(s->bdb.p + static_cast<size_t>(n0) * Np, BDc, Bd.p, Np, Nf, nb, st);
             Gemm g;   // S0 -= (B D^{-1}) B^T
             g.M = Np; g.N = Np; g.K = Nf;
-            g.A = s->BD.p + static_cast<size_t>(n0) * pf; g.lda = Np; g.sA = pf;
+            g.A = BDc; g.lda = Np; g.sA = pf;
             g.B = Bd.p; g.ldb = Np; g.sB = pf; g.tB = 1;
             g.C = S0c; g.ldc = Np; g.sC = pp;
             g.alpha = -1.0; g.beta = 1.0; g.batch = nb;
@@ -276,6 +296,7 @@ void build_shared(sehf_scan* s) {
     if (on_host)
         SEHF_HIP_CHECK(hipMemcpyAsync(s->cvd.p, cvd_h.data(), cvd_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
+    if (!streamed) s->tin = sehf_scan::TableInputs();   // only the per-step rebuild needs them
     // Gershgorin bounds of A0_n and S0_n (SchurInverter.cpp:30-49)
     auto gersh = [Np](const std::vector<double>& M) {
         double lb = 1e300;
@@ -316,6 +337,31 @@ void build_shared(sehf_scan* s) {
     SEHF_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// Streamed tables: S0_n and, for n < n_bd, B_n D_n^{-1} (into BDout) rebuilt
+// on the device for one radial point, with the resident build's kernels in
+// its order (bitwise the same values).  S0_n goes to b.tS0.
+void tables_at(sehf_scan* s, Batch& b, int n, double* BDout, hipStream_t st) {
+    const Setup& S = *s->S;
+    auto& T = s->tin;
+    const int Np = s->Np, Nf = s->Nf;
+    const size_t pp = static_cast<size_t>(Np) * Np, pf = static_cast<size_t>(Np) * Nf;
+    build_V(b.tV.p, T.vs.p + static_cast<size_t>(n) * S.n_exp, S.n_exp, T.pi.p, T.pj.p, T.vptr.p, T.vsig.p, T.vval.p,
+            T.npairs, Np, T.lmu.p, T.vp.p + n, T.ir2.p + n, T.live.p + n, 1, st);
+    axpbyi(b.tS0.p, Np, pp, b.tV.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, 1, st);
+    if (Nf > 0 && n < s->n_bd) {
+        build_B(b.tB.p, T.chi.p + static_cast<size_t>(n) * S.n_occ * S.n_lambda, S.n_occ, S.n_lambda, T.xptr.p,
+                T.xlam.p, T.xval.p, Np, S.n_sigma, T.fac.p + n, s->h * s->h / 12.0, 1, st);
+        col_scale(BDout, Np, b.tB.p, Np, s->Dinv.p + static_cast<size_t>(n) * Nf, Np, Nf, st);
+        Gemm g;   // S0 -= (B D^{-1}) B^T
+        g.M = Np; g.N = Np; g.K = Nf;
+        g.A = BDout; g.lda = Np; g.sA = pf;
+        g.B = b.tB.p; g.ldb = Np; g.sB = pf; g.tB = 1;
+        g.C = b.tS0.p; g.ldc = Np; g.sC = pp;
+        g.alpha = -1.0; g.beta = 1.0; g.batch = 1;
+        gemm(g, st);
+    }
+}
+
 double lmin_A(sehf_scan* s, int n) {
     std::lock_guard<std::mutex> lock(s->host_mu);
     if (!s->hasLA[n]) { s->lA[n] = sym_min_eig(s->Np, s->A0h[n].data()); s->hasLA[n] = 1; }
@@ -345,56 +391,114 @@ void sinv_shift(sehf_scan* s, int n, double E, int& shA, int& shS, double& dA_ou
     if (meS < thr) { dS_out = thr - meS + Setup::shift_margin; ++shS; }
 }
 
+}  // namespace
+
+namespace sehf {
+
+// Bytes of the Gauss-Jordan workspace of `batch` matrices of order n.
+size_t gj_buffer_bytes(size_t n, size_t batch) {
+    return batch * (sizeof(int) * (2 * n + 2 + gj_nom_elems(static_cast<int>(n)) + gj_stgi_elems()) +
+                    sizeof(double) * (gj_tbuf_elems(static_cast<int>(n)) + gj_stg_elems()));
+}
+
+// The batch buffers of the scan, for `B` energies with checkpoint spacing
+// L and n_ck checkpoints per energy: `al(array, count, in_host_memory)` is
+// called once per buffer.  alloc_batch allocates through it and the
+// storage planner counts through it, so the two cannot disagree.
+template <class F>
+void batch_arrays(Batch& b, const BatchDims& d, int B, int L, int n_ck, int n_fit, bool tan, F&& al) {
+    const size_t Np = d.Np, Nf = d.Nf, Nt = d.Nt, ns = d.ns;
+    const size_t tt = Nt * Nt, pp = Np * Np, pf = Np * std::max<size_t>(Nf, 1), ff = std::max<size_t>(Nf, 1) * std::max<size_t>(Nf, 1);
+    const bool hc = d.ckpt_host;
+    al(b.Rp, tt * B, false); al(b.Rc, tt * B, false); al(b.U, tt * B, false);
+    al(b.Ms, pp * B, false); al(b.X, pf * B, false);
+    al(b.Z, Nt * Np * B, false); al(b.Zt, Nt * Np * B, false);
+    al(b.T, pp * B, false); al(b.Psi, pp * B, false); al(b.acc, ns * Np * B, false);
+    al(b.YJ, pp * B, false); al(b.YY, pp * B, false); al(b.Am, pp * B, false); al(b.Bm, pp * B, false);
+    al(b.Kd, pp * B, false);
+    al(b.P, pp * B, false); al(b.Vpsi, pp * B, false); al(b.G, pf * B, false); al(b.Bf, pp * B, false);
+    al(b.ckpt, tt * B * std::max(n_ck, 1), hc);
+    al(b.segR, tt * B * L, false); al(b.segS, pp * B * L, false);
+    al(b.W2, 4 * pp * B, false); al(b.W2i, 4 * pp * B, false);
+    al(b.Rff, ff * B, false); al(b.Rffi, ff * B, false); al(b.T1, pf * B, false);
+    al(b.DRre, 6 * Np * B, false); al(b.DRim, 6 * Np * B, false);
+    al(b.ore, Np * d.n_occ * B, false); al(b.oim, Np * d.n_occ * B, false);
+    al(b.Dre, 6 * Np * B, false); al(b.Dim, 6 * Np * B, false); al(b.Ainv, pp * B, false); al(b.Kfit, pp * B, false);
+    al(b.dA, (d.nstab + 1) * B, false); al(b.dS, (d.nstab + 1) * B, false);
+    al(b.zeros, static_cast<size_t>(B), false); al(b.cE, static_cast<size_t>(B), false);
+    al(b.anl, (d.n_start + 1) * B * Nt, false);
+    al(b.tab, 6 * Np * B, false); al(b.jy, 2 * Np * B, false);
+    al(b.Jc, std::max<size_t>(n_fit, 1) * Np * B, false); al(b.Yc, std::max<size_t>(n_fit, 1) * Np * B, false);
+    al(b.coef, 5 * Np * B, false); al(b.red, 2 * static_cast<size_t>(B), false); al(b.nrm, 2 * static_cast<size_t>(B), false);
+    al(b.SinvR, 2 * pp * B, false);
+    al(b.Ad, ns * Nt * B, false); al(b.Ad2, ns * Nt * B, false);
+    if (d.streamed) {
+        al(b.tS0, pp, false); al(b.tV, pp, false); al(b.tB, pf, false);
+        al(b.ringBD, 2 * pf, false); al(b.segBD, pf * L, false);
+    }
+    if (tan) {
+        const size_t no = d.n_occ, nx = std::max<size_t>(6, no);
+        al(b.dRp, tt * B, false); al(b.dRc, tt * B, false);
+        al(b.dckpt, tt * B * std::max(n_ck, 1), hc); al(b.dsegR, tt * B * L, false);
+        al(b.Gt, Np * Nt * B, false); al(b.dZ, Nt * Np * B, false); al(b.dZt, Nt * Np * B, false);
+        al(b.P1, pp * B, false); al(b.dPsi, pp * B, false); al(b.dacc, ns * Np * B, false);
+        al(b.dYJ, pp * B, false); al(b.dYY, pp * B, false); al(b.dAm, pp * B, false); al(b.dBm, pp * B, false);
+        al(b.dKd, pp * B, false);
+        al(b.dP, pp * B, false); al(b.Q1, pp * B, false); al(b.Q2, pp * B, false);
+        al(b.vre, Np * nx * B, false); al(b.vim, Np * nx * B, false);
+        al(b.dDRre, 6 * Np * B, false); al(b.dDRim, 6 * Np * B, false);
+        al(b.dore, Np * no * B, false); al(b.doim, Np * no * B, false);
+        al(b.dDre, 6 * Np * B, false); al(b.dDim, 6 * Np * B, false);
+        al(b.wF, d.Nr * B, false); al(b.wP, d.Nr * B, false); al(b.wN, static_cast<size_t>(B), false);
+        al(b.danl, (d.n_start + 1) * B * Nt, false);
+        al(b.dtab, 4 * Np * B, false); al(b.djy, 2 * Np * B, false);
+        al(b.dJc, std::max<size_t>(n_fit, 1) * Np * B, false); al(b.dYc, std::max<size_t>(n_fit, 1) * Np * B, false);
+        al(b.dcoef, 4 * Np * B, false);
+    }
+}
+
+BatchBytes batch_bytes(const BatchDims& d, int B, int L, int n_ck, int n_fit, bool tan) {
+    Batch b;
+    BatchBytes out;
+    batch_arrays(b, d, B, L, n_ck, n_fit, tan, [&](auto& arr, size_t count, bool host) {
+        (host ? out.host : out.device) += count * sizeof(*arr.p);
+    });
+    out.device += gj_buffer_bytes(std::max(d.Nt, 2 * d.Np), B) + gj_buffer_bytes(d.Np, B);
+    return out;
+}
+
+}  // namespace sehf
+
+namespace {
+
+BatchDims dims_of(const sehf_scan* s) {
+    BatchDims d;
+    d.Np = s->Np; d.Nf = s->Nf; d.Nt = s->Nt; d.ns = s->nslot; d.n_occ = s->S->n_occ;
+    d.nstab = s->nstab; d.n_start = s->n_start; d.Nr = s->Nr;
+    d.streamed = s->tables == SEHF_TABLES_STREAMED;
+    d.ckpt_host = s->ckpt_where == SEHF_CKPT_HOST;
+    return d;
+}
+
 // n_ck: segment start matrices (checkpoints) kept per energy.
 void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_ck, int n_fit, bool tan) {
     Batch& b = g.bt;
-    if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt && b.ckpt.n >= static_cast<size_t>(std::max(n_ck, 1)) * B * s->Nt * s->Nt &&
-        b.Jc.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np && (!tan || b.has_tan))
+    const BatchDims d = dims_of(s);
+    if (b.B >= B && b.segR.n >= static_cast<size_t>(L) * B * s->Nt * s->Nt &&
+        b.ckpt.n >= static_cast<size_t>(std::max(n_ck, 1)) * B * s->Nt * s->Nt &&
+        b.Jc.n >= static_cast<size_t>(std::max(n_fit, 1)) * B * s->Np && (!tan || b.has_tan) &&
+        b.ckpt.host == d.ckpt_host && (!d.streamed || b.segBD.n >= static_cast<size_t>(L) * s->Np * std::max(s->Nf, 1)))
         return;
-    const size_t Np = s->Np, Nf = s->Nf, Nt = s->Nt, ns = s->nslot;
-    const size_t tt = Nt * Nt, pp = Np * Np, pf = Np * std::max<size_t>(Nf, 1), ff = std::max<size_t>(Nf, 1) * std::max<size_t>(Nf, 1);
     b = Batch();
     b.B = B;
-    b.Rp.alloc(tt * B); b.Rc.alloc(tt * B); b.U.alloc(tt * B);
-    b.Ms.alloc(pp * B); b.X.alloc(pf * B);
-    b.Z.alloc(Nt * Np * B); b.Zt.alloc(Nt * Np * B);
-    b.T.alloc(pp * B); b.Psi.alloc(pp * B); b.acc.alloc(ns * Np * B);
-    b.YJ.alloc(pp * B); b.YY.alloc(pp * B); b.Am.alloc(pp * B); b.Bm.alloc(pp * B); b.Kd.alloc(pp * B);
-    b.P.alloc(pp * B); b.Vpsi.alloc(pp * B); b.G.alloc(pf * B); b.Bf.alloc(pp * B);
-    b.ckpt.alloc(tt * B * std::max(n_ck, 1));
-    b.segR.alloc(tt * B * L); b.segS.alloc(pp * B * L);
-    b.W2.alloc(4 * pp * B); b.W2i.alloc(4 * pp * B);
-    b.Rff.alloc(ff * B); b.Rffi.alloc(ff * B); b.T1.alloc(pf * B);
-    b.DRre.alloc(6 * Np * B); b.DRim.alloc(6 * Np * B); b.ore.alloc(Np * s->S->n_occ * B); b.oim.alloc(Np * s->S->n_occ * B);
-    b.Dre.alloc(6 * Np * B); b.Dim.alloc(6 * Np * B); b.Ainv.alloc(pp * B); b.Kfit.alloc(pp * B);
-    b.dA.alloc(static_cast<size_t>(s->nstab + 1) * B); b.dS.alloc(static_cast<size_t>(s->nstab + 1) * B);
-    b.zeros.alloc(B); fill(b.zeros.p, B, 0.0, g.st); b.cE.alloc(B);
-    b.anl.alloc(static_cast<size_t>(s->n_start + 1) * B * Nt);
-    b.tab.alloc(6 * Np * B); b.jy.alloc(2 * Np * B);
-    b.Jc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.Yc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
-    b.coef.alloc(5 * Np * B); b.red.alloc(2 * B); b.nrm.alloc(2 * B);
-    b.gj.ensure(static_cast<int>(std::max(Nt, 2 * Np)), B);
-    b.gj2.ensure(static_cast<int>(Np), B);
-    b.SinvR.alloc(2 * pp * B);
-    b.Ad.alloc(ns * Nt * B); b.Ad2.alloc(ns * Nt * B);
-    if (tan) {
-        const size_t no = s->S->n_occ, nx = std::max<size_t>(6, no);
-        b.has_tan = true;
-        b.dRp.alloc(tt * B); b.dRc.alloc(tt * B);
-        b.dckpt.alloc(tt * B * std::max(n_ck, 1)); b.dsegR.alloc(tt * B * L);
-        b.Gt.alloc(Np * Nt * B); b.dZ.alloc(Nt * Np * B); b.dZt.alloc(Nt * Np * B);
-        b.P1.alloc(pp * B); b.dPsi.alloc(pp * B); b.dacc.alloc(ns * Np * B);
-        b.dYJ.alloc(pp * B); b.dYY.alloc(pp * B); b.dAm.alloc(pp * B); b.dBm.alloc(pp * B); b.dKd.alloc(pp * B);
-        b.dP.alloc(pp * B); b.Q1.alloc(pp * B); b.Q2.alloc(pp * B);
-        b.vre.alloc(Np * nx * B); b.vim.alloc(Np * nx * B);
-        b.dDRre.alloc(6 * Np * B); b.dDRim.alloc(6 * Np * B); b.dore.alloc(Np * no * B); b.doim.alloc(Np * no * B);
-        b.dDre.alloc(6 * Np * B); b.dDim.alloc(6 * Np * B);
-        b.wF.alloc(static_cast<size_t>(s->Nr) * B); b.wP.alloc(static_cast<size_t>(s->Nr) * B); b.wN.alloc(B);
-        b.danl.alloc(static_cast<size_t>(s->n_start + 1) * B * Nt);
-        b.dtab.alloc(4 * Np * B); b.djy.alloc(2 * Np * B);
-        b.dJc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B); b.dYc.alloc(static_cast<size_t>(std::max(n_fit, 1)) * Np * B);
-        b.dcoef.alloc(4 * Np * B);
-    }
+    batch_arrays(b, d, B, L, n_ck, n_fit, tan, [](auto& arr, size_t count, bool host) {
+        if (host) arr.alloc_host(count);
+        else arr.alloc(count);
+    });
+    fill(b.zeros.p, B, 0.0, g.st);
+    b.gj.ensure(static_cast<int>(std::max(d.Nt, 2 * d.Np)), B);
+    b.gj2.ensure(static_cast<int>(d.Np), B);
+    b.has_tan = tan;
     g.ensure_seg_events(L);
 }
 
@@ -695,28 +799,36 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // A_n within a few percent of I beyond), so by default it is inverted in
     // place by block elimination without pivoting; SEHF_SINV_INVERSE=gj
     // selects the pivoted Gauss-Jordan inverse instead.
-    auto sinv_step = [&](int n, double* Sout) {
+    // Streamed tables: the side stream rebuilds S0_n (and B_n D_n^{-1} into
+    // bd_out, where the consumers of point n read it) before the Schur inverse.
+    const bool streamed = s->tables == SEHF_TABLES_STREAMED;
+    auto sinv_step = [&](int n, double* Sout, double* bd_out) {
         const double* a = (n < s->nstab) ? b.dA.p + static_cast<size_t>(n) * nb : b.zeros.p;
         const double* d = (n < s->nstab) ? b.dS.p + static_cast<size_t>(n) * nb : b.zeros.p;
         double* Sm = s->sinv_spd ? Sout : b.Ms.p;
-        sinv_matrix(Sm, s->S0.p + static_cast<size_t>(n) * pp, s->cvd.p + static_cast<size_t>(n) * Np,
+        const double* S0n = streamed ? nullptr : s->S0.p + static_cast<size_t>(n) * pp;
+        if (streamed) {
+            tables_at(s, b, n, bd_out, st2);
+            S0n = b.tS0.p;
+        }
+        sinv_matrix(Sm, S0n, s->cvd.p + static_cast<size_t>(n) * Np,
                     s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st2);
         if (s->sinv_spd) spd_inverse(Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
         else gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
     };
-    auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout) {
+    auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout, const double* BDn) {
         init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
         if (n < s->n_tr && Nf > 0) {
             Gemm g;   // Xn = -Sinv (B D^{-1})
             g.M = Np; g.N = Nf; g.K = Np;
             g.A = Sv; g.lda = Np; g.sA = pp;
-            g.B = s->BD.p + static_cast<size_t>(n) * pf; g.ldb = Np; g.sB = 0;
+            g.B = BDn; g.ldb = Np; g.sB = 0;
             g.C = Xd; g.ldc = Np; g.sC = sXd; g.alpha = -1.0; g.batch = nb;
             gemm(g, st);
             add_psi_f_blocks(b.U.p, Xd, sXd, 12.0, Np, Nf, Nt, nb, st);
             Gemm h2;  // U_ff -= 12 (B D^{-1})^T Xn
             h2.M = Nf; h2.N = Nf; h2.K = Np;
-            h2.A = s->BD.p + static_cast<size_t>(n) * pf; h2.lda = Np; h2.sA = 0; h2.tA = 1;
+            h2.A = BDn; h2.lda = Np; h2.sA = 0; h2.tA = 1;
             h2.B = Xd; h2.ldb = Np; h2.sB = sXd;
             h2.C = b.U.p + Np + static_cast<size_t>(Np) * Nt; h2.ldc = Nt; h2.sC = tt;
             h2.alpha = -12.0; h2.beta = 1.0; h2.batch = nb;
@@ -800,10 +912,14 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // rows there still need those points' Sinv_n, so the loop starts at n = 1.
     const bool adj = s->dipole_path == SEHF_DIPOLES_FORWARD && !tan;
     const int n_lo = (adj && seg < 0 && probe == 0) ? std::min(1, n_first) : n_first;
+    // B_n D_n^{-1} of the forward sweep: resident, or the ring slot of point n
+    auto bd_fwd = [&](int n) -> double* {
+        return streamed ? b.ringBD.p + static_cast<size_t>(n & 1) * pf : s->BD.p + static_cast<size_t>(n) * pf;
+    };
     auto sinv_ahead = [&](int n) {
         const int slot = n & 1;
         if (n - 2 >= n_lo) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
-        sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb);
+        sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb, bd_fwd(n));
         SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
     };
     const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
@@ -842,7 +958,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 Gemm g;   // Xn = -Sinv (B D^{-1})
                 g.M = Np; g.N = Nf; g.K = Np;
                 g.A = Sslot; g.lda = Np; g.sA = pp;
-                g.B = s->BD.p + static_cast<size_t>(n) * pf; g.ldb = Np; g.sB = 0;
+                g.B = bd_fwd(n); g.ldb = Np; g.sB = 0;
                 g.C = Xd; g.ldc = Np; g.sC = sXd; g.alpha = -1.0; g.batch = nb;
                 gemm(g, st);
             }
@@ -851,7 +967,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
             continue;
         }
-        frp_step(n, Sslot, b.Rp.p, b.Rc.p);
+        frp_step(n, Sslot, b.Rp.p, b.Rc.p, bd_fwd(n));
         if (adj) adj_step(n, Sslot, b.Rc.p);
         if (tan) frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);
         SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
@@ -861,11 +977,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if ((n + 1) % L == 0) {
             const int sidx = (n + 1) / L;
             if (sidx < n_seg && sidx >= ck_lo) {
+                // (checkpoints may be in pinned host memory: hipMemcpyDefault)
                 SEHF_HIP_CHECK(hipMemcpyAsync(ckp(b.ckpt.p, sidx), b.Rp.p, nb * tt * sizeof(double),
-                                              hipMemcpyDeviceToDevice, st));
+                                              hipMemcpyDefault, st));
                 if (tan)
                     SEHF_HIP_CHECK(hipMemcpyAsync(ckp(b.dckpt.p, sidx), b.dRp.p, nb * tt * sizeof(double),
-                                                  hipMemcpyDeviceToDevice, st));
+                                                  hipMemcpyDefault, st));
             }
         }
     }
@@ -1029,12 +1146,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // psi-block of W_n^{-1} Zin (WInverseOperator.cpp:75-128):
     //   out = Sinv_n (Z_psi - (B D^{-1}) Z_f)
     const int64_t sZ = static_cast<int64_t>(Nt) * Np;
-    auto winv_psi = [&](int n, const double* Zin, const double* Sn, double* outp) {
+    auto winv_psi = [&](int n, const double* Zin, const double* Sn, double* outp, const double* BDn) {
         get_block(b.T.p, Zin, Nt, sZ, 0, 0, Np, Np, nb, st);
         if (n < s->n_tr && Nf > 0) {
             Gemm g2;  // rhs = Z_psi - (B D^{-1}) Z_f
             g2.M = Np; g2.N = Np; g2.K = Nf;
-            g2.A = s->BD.p + static_cast<size_t>(n) * pf; g2.lda = Np; g2.sA = 0;
+            g2.A = BDn; g2.lda = Np; g2.sA = 0;
             g2.B = Zin + Np; g2.ldb = Nt; g2.sB = sZ;
             g2.C = b.T.p; g2.ldc = Np; g2.sC = pp; g2.alpha = -1.0; g2.beta = 1.0; g2.batch = nb;
             gemm(g2, st);
@@ -1081,11 +1198,16 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             // handful of inverses instead of the whole segment's (each is a
             // few hundred launches).
             if (seg_pending) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_bwd, 0));
+            // B_n D_n^{-1} of the segment's points: resident, or rebuilt with
+            // the segment's Schur inverses into segBD
+            auto bd_seg = [&](int n) -> double* {
+                return streamed ? b.segBD.p + static_cast<size_t>(n - n0) * pf : s->BD.p + static_cast<size_t>(n) * pf;
+            };
             constexpr int kAhead = 4;
             int n_enq = n0;
             auto enqueue_sinv_to = [&](int n_hi) {
                 for (; n_enq < std::min(n_hi, n1); ++n_enq) {
-                    sinv_step(n_enq, b.segS.p + static_cast<size_t>(n_enq - n0) * nb * pp);
+                    sinv_step(n_enq, b.segS.p + static_cast<size_t>(n_enq - n0) * nb * pp, bd_seg(n_enq));
                     SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n_enq - n0], st2));
                 }
             };
@@ -1097,9 +1219,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 double* dRn = tg ? b.dsegR.p + static_cast<size_t>(n - n0) * nb * tt : nullptr;
                 SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_seg[n - n0], 0));
                 if (n >= ns0) {
-                    frp_step(n, Sn, prev, Rn);
+                    frp_step(n, Sn, prev, Rn, bd_seg(n));
                     if (tg) {   // dRinv_{n-1} is kept: the tangent works on a copy
-                        SEHF_HIP_CHECK(hipMemcpyAsync(b.dRp.p, dprev, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
+                        SEHF_HIP_CHECK(hipMemcpyAsync(b.dRp.p, dprev, nb * tt * sizeof(double), hipMemcpyDefault, st));
                         frp_tangent(n, Sn, Rn, b.dRp.p, dRn);
                     }
                 } else if (n == 0) {
@@ -1126,7 +1248,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
                 mm(b.Zt.p, Rn, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);   // Z <- Rinv_n Z
                 std::swap(b.Z.p, b.Zt.p);
-                winv_psi(n, b.Z.p, Sn, b.Psi.p);                                     // psi_n
+                winv_psi(n, b.Z.p, Sn, b.Psi.p, bd_seg(n));                          // psi_n
                 const bool in_win = n >= n_fs && n <= n_fe;
                 if (main && n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
                 if (in_win) {
@@ -1150,7 +1272,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                     if (n < s->n_tr && Nf > 0) {
                         Gemm g;   // dZ_f -= wF_n (B D^{-1})^T P1
                         g.M = Nf; g.N = Np; g.K = Np;
-                        g.A = s->BD.p + static_cast<size_t>(n) * pf; g.lda = Np; g.sA = 0; g.tA = 1;
+                        g.A = bd_seg(n); g.lda = Np; g.sA = 0; g.tA = 1;
                         g.B = b.P1.p; g.ldb = Np; g.sB = pp;
                         g.C = b.dZ.p + Np; g.ldc = Nt; g.sC = sZ;
                         g.alpha = -1.0; g.alpha_b = wFn; g.beta = 1.0; g.batch = nb;
@@ -1164,7 +1286,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
                 }
                 std::swap(b.dZ.p, b.dZt.p);
                 // dpsi_n = (W_n^{-1} dZ_n)_psi + wP_n Sinv_n psi_n
-                winv_psi(n, b.dZ.p, Sn, b.dPsi.p);
+                winv_psi(n, b.dZ.p, Sn, b.dPsi.p, bd_seg(n));
                 axpy_b(b.dPsi.p, Np, pp, b.P1.p, Np, pp, b.wP.p + static_cast<size_t>(n) * nb, 1.0, Np, Np, nb, st);
                 if (main && n < s->n_pts) dip_accum(n, b.dPsi.p, b.dacc.p);
                 if (in_win) fit_tangent(n, b.Psi.p, b.dPsi.p);
@@ -1362,8 +1484,17 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
 extern "C" {
 
 int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
+    const char* e = std::getenv("SEHF_TABLES");
+    const int tables = (e && std::string(e) == "streamed") ? SEHF_TABLES_STREAMED : SEHF_TABLES_RESIDENT;
+    return sehf_scan_create_with(ctx, p, tables, out);
+}
+
+int sehf_scan_create_with(sehf_ctx* ctx, const sehf_problem* p, int tables, sehf_scan** out) {
     return guarded([&] {
+        if (tables != SEHF_TABLES_RESIDENT && tables != SEHF_TABLES_STREAMED)
+            throw std::runtime_error("sehf_scan_create_with: tables must be SEHF_TABLES_RESIDENT or _STREAMED");
         auto s = std::make_unique<sehf_scan>();
+        s->tables = tables;
         s->ctx = ctx;
         s->st = sehf_ctx_stream(ctx);
         s->ctx_device = sehf_ctx_device(ctx);
@@ -1379,6 +1510,35 @@ int sehf_scan_create(sehf_ctx* ctx, const sehf_problem* p, sehf_scan** out) {
         build_shared(s.get());
         s->t_stage[0] = wall() - t0;
         *out = s.release();
+    });
+}
+
+int sehf_scan_set_checkpoints(sehf_scan* s, int where) {
+    return guarded([&] {
+        if (where != SEHF_CKPT_DEVICE && where != SEHF_CKPT_HOST)
+            throw std::runtime_error("sehf_scan_set_checkpoints: where must be SEHF_CKPT_DEVICE or _HOST");
+        s->ckpt_where = where;
+    });
+}
+
+int sehf_scan_memory(const sehf_scan* s, size_t* bytes) {
+    return guarded([&] {
+        size_t sh = s->S0.bytes() + s->BD.bytes() + s->Dinv.bytes() + s->VN.bytes() + s->BN.bytes() +
+                    s->negDinvN.bytes() + s->Drows.bytes() + s->dcorr.bytes() + s->cvd.bytes() + s->bdb.bytes();
+        const auto& T = s->tin;
+        sh += T.pi.bytes() + T.pj.bytes() + T.vsig.bytes() + T.lmu.bytes() + T.xlam.bytes() + T.vptr.bytes() +
+              T.xptr.bytes() + T.vval.bytes() + T.xval.bytes() + T.vs.bytes() + T.vp.bytes() + T.ir2.bytes() +
+              T.chi.bytes() + T.fac.bytes() + T.live.bytes();
+        size_t dev = 0, host = 0;
+        for (const auto& g : s->groups) {
+            Batch& b = const_cast<Batch&>(g->bt);
+            BatchDims d = dims_of(s);
+            batch_arrays(b, d, 1, 1, 1, 1, b.has_tan, [&](auto& arr, size_t, bool) {
+                (arr.host ? host : dev) += arr.bytes();
+            });
+            dev += gj_buffer_bytes(b.gj.n, b.gj.batch) + gj_buffer_bytes(b.gj2.n, b.gj2.batch);
+        }
+        bytes[0] = sh; bytes[1] = dev; bytes[2] = host;
     });
 }
 

@@ -35,6 +35,46 @@ class TangentOut_C(ctypes.Structure):
     _fields_ = [(n, _dp) for n in ("dK", "dA", "dB", "db_overlap", "dd_raw", "dD")]
 
 
+class PlanIn_C(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("Nr", "n_psi", "n_f", "n_transition", "n_occ", "n_exp", "n_lambda",
+                                            "tangent", "dipole_path", "n_energies", "batch_cap")] + \
+               [(n, ctypes.c_size_t) for n in ("device_bytes", "host_bytes", "runtime_bytes")] + \
+               [("reserve_fraction", ctypes.c_double)]
+
+
+class Plan_C(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("tables", "ckpt", "seg_len", "n_ckpt", "batch")] + \
+               [(n, ctypes.c_size_t) for n in ("budget_bytes", "shared_bytes", "per_energy_bytes",
+                                               "host_per_energy_bytes")] + \
+               [("report", ctypes.c_char * 1536)]
+
+
+TABLES = {"resident": 0, "streamed": 1}      # SEHF_TABLES_*
+CHECKPOINTS = {"device": 0, "host": 1}       # SEHF_CKPT_*
+
+
+def plan_storage(problem, n_energies: int, tangent: bool = False, dipole_path: int = 0, device_bytes: int = 0,
+                 host_bytes: int = 0, runtime_bytes: int = 2 << 30, reserve_fraction: float = 0.10,
+                 batch_cap: int = 0, n_transition: int | None = None) -> dict:
+    """Device storage plan for scanning `n_energies` energies of `problem`
+    (sehf_plan_storage): tables resident or streamed, checkpoint level and
+    spacing, batch width, and the report.  n_transition defaults to Nr (the
+    exchange block kept over the whole grid, the upper bound)."""
+    L = _register(load())
+    p = problem
+    q = PlanIn_C(p.Nr, p.n_psi, p.n_f, p.Nr if n_transition is None else int(n_transition), p.n_occ,
+                 (2 * p.l_cont + 1) ** 2, p.n_lambda, 1 if tangent else 0, int(dipole_path), int(n_energies),
+                 int(batch_cap), int(device_bytes), int(host_bytes), int(runtime_bytes), float(reserve_fraction))
+    out = Plan_C()
+    check(L.sehf_plan_storage(ctypes.byref(q), ctypes.byref(out)), "sehf_plan_storage")
+    inv_t = {v: k for k, v in TABLES.items()}
+    inv_c = {v: k for k, v in CHECKPOINTS.items()}
+    return {"tables": inv_t[out.tables], "checkpoints": inv_c[out.ckpt], "seg_len": out.seg_len,
+            "n_ckpt": out.n_ckpt, "batch": out.batch, "budget_bytes": out.budget_bytes,
+            "shared_bytes": out.shared_bytes, "per_energy_bytes": out.per_energy_bytes,
+            "host_per_energy_bytes": out.host_per_energy_bytes, "report": out.report.decode()}
+
+
 def _register(L):
     if getattr(L, "_scan_registered", False):
         return L
@@ -50,6 +90,15 @@ def _register(L):
     L.sehf_scan_run_tangent.restype = ctypes.c_int
     L.sehf_scan_destroy.argtypes = [ctypes.c_void_p]
     L.sehf_scan_destroy.restype = None
+    L.sehf_scan_create_with.argtypes = [ctypes.c_void_p, ctypes.POINTER(Problem_C), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+    L.sehf_scan_create_with.restype = ctypes.c_int
+    L.sehf_scan_set_checkpoints.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.sehf_scan_set_checkpoints.restype = ctypes.c_int
+    L.sehf_scan_memory.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+    L.sehf_scan_memory.restype = ctypes.c_int
+    L.sehf_plan_storage.argtypes = [ctypes.POINTER(PlanIn_C), ctypes.POINTER(Plan_C)]
+    L.sehf_plan_storage.restype = ctypes.c_int
     L._scan_registered = True
     return L
 
@@ -73,7 +122,9 @@ def _c_problem(p, keep):
 class Scan:
     """Energy-independent device state for one molecule (sehf_scan handle)."""
 
-    def __init__(self, problem, ctx=None):
+    def __init__(self, problem, ctx=None, tables: str | None = None):
+        """tables: None (sehf_scan_create: resident unless $SEHF_TABLES=streamed),
+        "resident" or "streamed" (sehf_scan_create_with)."""
         from .steppers import GpuContext
         L = _register(load())
         self.ctx = ctx or GpuContext(True)
@@ -81,7 +132,11 @@ class Scan:
         keep = []
         cp = _c_problem(problem, keep)
         h = ctypes.c_void_p()
-        check(L.sehf_scan_create(self.ctx._h, ctypes.byref(cp), ctypes.byref(h)), "sehf_scan_create")
+        if tables is None:
+            check(L.sehf_scan_create(self.ctx._h, ctypes.byref(cp), ctypes.byref(h)), "sehf_scan_create")
+        else:
+            check(L.sehf_scan_create_with(self.ctx._h, ctypes.byref(cp), TABLES[tables], ctypes.byref(h)),
+                  "sehf_scan_create_with")
         self._h = h
         info = (ctypes.c_int * 8)()
         check(L.sehf_scan_info(h, info))
@@ -133,6 +188,18 @@ class Scan:
                           "db_overlap": o["db_overlap"][e].T.copy(), "dd_raw": o["dd_raw"][e], "dD": o["dD"][e]})
             res.append(r)
         return res
+
+    def set_checkpoints(self, where: str):
+        """Ratio-matrix checkpoints in "device" memory or pinned "host"
+        memory (sehf_scan_set_checkpoints)."""
+        check(load().sehf_scan_set_checkpoints(self._h, CHECKPOINTS[where]), "sehf_scan_set_checkpoints")
+
+    def memory(self) -> dict:
+        """Bytes held: energy-independent device tables, batch buffers, pinned
+        host checkpoints (sehf_scan_memory)."""
+        b = (ctypes.c_size_t * 3)()
+        check(load().sehf_scan_memory(self._h, b), "sehf_scan_memory")
+        return {"shared": b[0], "batch": b[1], "host": b[2]}
 
     DIPOLES_FORWARD = 0    # SEHF_DIPOLES_FORWARD
     DIPOLES_BACKWARD = 1   # SEHF_DIPOLES_BACKWARD
