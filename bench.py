@@ -50,13 +50,14 @@ sys.path.insert(0, os.path.join(ROOT, "spion-emol_amd"))
 PEAK_FP64_MFMA_TFLOPS = 78.6      # MI355X dense FP64 matrix peak (datasheet)
 METRIC = "energy points/sec @l_cont=30 (batched vs per-ik, 1 GPU); FP64-MFMA %; adjoint/fwd cost"
 
-# HBM bytes of one k_dgemm launch of the top-level Gauss-Jordan update of the
-# order-1566 inverse (783 x 1566 x 783, the largest single share of GEMM time
-# at l_cont = 30): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
+# HBM bytes of the main k_dgemm launch (768 rows, 128 x 128 tiles) of the
+# top-level Gauss-Jordan update of the order-1566 inverse (783 x 1566 x 783,
+# the largest single share of GEMM time at l_cont = 30): rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE per dispatch over `bench.py --kernels gemm`,
 # scaled by the same counters on a 1 GiB device copy (FETCH x 2, WRITE x 1).
-# batch -> (measured bytes, algorithmic bytes).  An external calibration run,
-# not a measurement of this process.
-TOP_UPDATE_TRAFFIC = {32: (1.7831e+09, 1.0987e+09), 48: (2.7581e+09, 1.6480e+09)}
+# batch -> (measured bytes, algorithmic bytes).  From separate counter runs of
+# the kernel table at the scan's shapes, not from this process.
+TOP_UPDATE_TRAFFIC = {64: (3.416e+09, 2.167e+09)}
 
 
 def log(msg):
@@ -258,9 +259,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=48,
-                    help="energies per batch (48 is the autotuner's choice on this configuration); 0 = run "
-                         "the library's autotuner first (timed probes, outside the timed region)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="energies per batch; 0 = the device storage planner's choice (sehf_plan_storage: the "
+                         "widest batch the HBM holds, at most 64); -1 = run the library's autotuner first (timed "
+                         "probes, outside the timed region)")
     ap.add_argument("--tiles", type=int, default=2, help="GEMM tile policy (sehf_set_gemm_tiles); -1 = library default")
     ap.add_argument("--config", default="h2o_l30")
     ap.add_argument("--cpu-steps", type=int, default=4, help="radial steps in the CPU oracle sample")
@@ -301,15 +303,18 @@ def main():
         f"(inputs generated in {time.time() - t0:.1f} s)")
     fwd_dip = args.dipole_path == "forward"
     dpath = sc.Scan.DIPOLES_FORWARD if fwd_dip else sc.Scan.DIPOLES_BACKWARD
+    nk = len(ks_all)
+    plan = sc.plan_storage(p, nk, dipole_path=dpath)
+    log("storage plan:\n" + plan["report"])
     t0 = time.time()
-    S = sc.Scan(p)
+    S = sc.Scan(p, tables=plan["tables"])
     S.set_dipole_path(dpath)
+    S.set_checkpoints(plan["checkpoints"])
     t_setup = time.time() - t0
     info = S.info
     log(f"setup (V(r), exchange, dipole tables on device) {t_setup:.1f} s; info {info}")
-    B, tune = args.batch, None
-    nk = len(ks_all)
-    if B <= 0:
+    B, tune = (plan["batch"] if args.batch == 0 else args.batch), None
+    if B < 0:
         t0 = time.time()
         tune = S.autotune(ks_all[(np.arange(min(128, nk)) * 37) % nk], probe_steps=8)
         B = tune["batch"]
@@ -427,12 +432,17 @@ def main():
         "ms_per_step": elapsed / max(args.steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "batch_energies": B, "segment_points": lay["L"], "n_segments": lay["n_seg"],
-                   "timed_segments": segs, "tiles": args.tiles, "dipole_path": args.dipole_path},
+                   "timed_segments": segs, "tiles": args.tiles, "dipole_path": args.dipole_path,
+                   "storage": {"tables": plan["tables"], "checkpoints": plan["checkpoints"],
+                               "planned_batch": plan["batch"], "batch_source": "plan" if args.batch == 0 else
+                               ("autotune" if args.batch < 0 else "argument")}},
         "roofline": {"bound": "mfma", "achieved": gemm_tflops, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": gemm_tflops / PEAK_FP64_MFMA_TFLOPS,
                      "traffic": TOP_UPDATE_TRAFFIC.get(B, (None,))[0] if args.config == "h2o_l30" else None,
-                     "traffic_note": "HBM bytes of one 783x1566x783 Gauss-Jordan update launch at this batch, from "
-                                     "a calibrated rocprofv3 --pmc run (external calibration, not this process)",
+                     "traffic_note": "HBM bytes of the 768-row main launch of the 783x1566x783 Gauss-Jordan update "
+                                     "at this batch (algorithmic %s), from calibrated rocprofv3 --pmc runs of "
+                                     "bench.py --kernels gemm (profiles/k_dgemm_counters_kernel_table_b64.md), not "
+                                     "this process" % (TOP_UPDATE_TRAFFIC.get(B, (None, None))[1],),
                      "kernel": "k_dgemm", "sampled_launches": int(prof[2]), "avg_launch_us": avg_launch_us,
                      "event_pair_overhead_us": prof[15] * 1e3},
         "model": model, "fixed_stages": fixed_info,
