@@ -140,16 +140,16 @@ def fixed_stage_seconds(p, ks, B):
     return fixed, {"short_grid_Nr": q.Nr, "scan_s": wall, "kmatrix_s": d["kmatrix"], "fit_dipole_s": d["fit_dipole"]}
 
 
-def cycle_rate(S, p, lay, ks, segs, warm, tangent=None):
+def cycle_rate(S, p, lay, ks, segs, warm, tangent=None, seg_len=0):
     """Segment cycles of the energies ks at the listed segments (after `warm`
     untimed ones); returns the modelled batch seconds and the timings."""
     N = p.Nr - 1
     for s in warm:
-        S.run_segment(ks, s, tangent=tangent)
+        S.run_segment(ks, s, seg_len=seg_len, tangent=tangent)
     tf, tr, tb, pts, walls = [], [], [], [], []
     for s in segs:
         t0 = time.time()
-        t = S.run_segment(ks, s, tangent=tangent)
+        t = S.run_segment(ks, s, seg_len=seg_len, tangent=tangent)
         walls.append(time.time() - t0)
         tf.append(t["forward"] + t["host_tables"]); tr.append(t["recompute"]); tb.append(t["back"])
         pts.append(seg_points(lay, N, s))
@@ -322,21 +322,22 @@ def main():
         log(f"autotune: batch {B}, groups {tune['groups']}, tiles {tune['tiles']} ({tune['seconds']:.1f} s)")
     # one batch: a spread of the scan's k values (closed and open channels mixed)
     ks = ks_all[(np.arange(B) * 37) % nk]
-    lay = S.layout()
+    L_plan = plan["seg_len"]
+    lay = S.layout(L_plan)
     N = p.Nr - 1
     warm = spread_segments(lay["n_seg"], args.warmup, offset=0.25)
     segs = spread_segments(lay["n_seg"], args.steps)
     log(f"layout {lay}; warmup segments {warm}; timed segments {segs}")
 
     for i, s in enumerate(warm):
-        S.run_segment(ks, s)
+        S.run_segment(ks, s, seg_len=L_plan)
         log(f"warmup step {i} (segment {s}) done")
     if args.prof_every > 0:
         check(L.sehf_prof_begin(args.prof_every), "sehf_prof_begin")
     tf, tr, tb, pts, walls = [], [], [], [], []
     t_start = time.time()
     for i, s in enumerate(segs):
-        t = S.run_segment(ks, s)
+        t = S.run_segment(ks, s, seg_len=L_plan)
         tf.append(t["forward"] + t["host_tables"]); tr.append(t["recompute"]); tb.append(t["back"])
         pts.append(seg_points(lay, N, s))
         log(f"timed step {i} (segment {s}, {pts[-1]} points) {t['host_tables'] + t['forward'] + t['recompute'] + t['back']:.2f} s "
@@ -361,7 +362,7 @@ def main():
         # energy per batch, the same segment cycles
         k1 = ks[B // 2: B // 2 + 1]
         few = spread_segments(lay["n_seg"], 4)
-        f1, r1, b1, p1, _ = cycle_rate(S, p, lay, k1, few, warm=[few[1]])
+        f1, r1, b1, p1, _ = cycle_rate(S, p, lay, k1, few, warm=[few[1]], seg_len=L_plan)
         t1, _ = batch_seconds(lay, N, p1, f1, r1, b1, 0.0, fwd_dip)
         side["per_ik"] = {"unit": "energy points/s", "segments": few, "modelled_energy_seconds": t1,
                           "value_without_fixed_stages": 1.0 / t1}
@@ -369,7 +370,7 @@ def main():
     full = None
     if args.full_scan:
         t0 = time.time()
-        res = S.run(ks, batch=B, groups=1)
+        res = S.run(ks, batch=B, seg_len=L_plan, groups=1)
         t_full = time.time() - t0
         if not all(np.all(np.isfinite(r["D"])) for r in res):
             raise SystemExit("non-finite dipole in the full scan")
@@ -412,7 +413,8 @@ def main():
                            "tangent_over_forward_cost": tb_t / tb_f}
         log(f"tangent / forward cost {tb_t / tb_f:.2f} on h2o_l20_pol")
 
-    workload = (f"{args.config}: H2O-like, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), Nr={p.Nr}, dr={p.dr}, "
+    molecule = {"h2o": "H2O-like", "ring": "16-centre ring", "h2": "H2-like"}.get(args.config.split("_")[0], p.name)
+    workload = (f"{args.config}: {molecule}, l_cont={p.l_cont} (N_psi={p.n_psi}, N_tot={p.n_tot}), Nr={p.Nr}, dr={p.dr}, "
                 f"k in [{ks_all[0]}, {ks_all[-1]}] ({nk} pts), "
                 + ("closed channels allowed" if p.allow_closed else "all channels open")
                 + f"; batch of {B} energies; dipoles "

@@ -248,21 +248,26 @@ void build_shared(sehf_scan* s) {
         }
         double* S0c = streamed ? S0k.p : s->S0.p + static_cast<size_t>(n0) * pp;
         double* BDc = streamed ? BDk.p : (Nf > 0 ? s->BD.p + static_cast<size_t>(n0) * pf : nullptr);
+        // Streamed tables keep S0 only for the stability-shift host copies
+        // (n < nstab); past them the chunk needs only B D^{-1} (for bdb).
+        const bool need_S0 = !streamed || n0 < s->nstab;
         // S0 = I - h^2/6 V
-        axpbyi(S0c, Np, pp, Vd.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, nc, st);
+        if (need_S0) axpbyi(S0c, Np, pp, Vd.p, Np, pp, -s->c6, nullptr, 0, 0, 0.0, 1.0, Np, Np, nc, st);
         if (Nf > 0 && nb > 0) {
             for (int q = 0; q < nb; ++q)
                 col_scale(BDc + q * pf, Np, Bd.p + q * pf, Np, s->Dinv.p + static_cast<size_t>(n0 + q) * Nf, Np, Nf,
                           st);
             // (B D^{-1} B^T)_ii as a separate dot product for the per-energy diagonal
             row_dot(s->bdb.p + static_cast<size_t>(n0) * Np, BDc, Bd.p, Np, Nf, nb, st);
-            Gemm g;   // S0 -= (B D^{-1}) B^T
-            g.M = Np; g.N = Np; g.K = Nf;
-            g.A = BDc; g.lda = Np; g.sA = pf;
-            g.B = Bd.p; g.ldb = Np; g.sB = pf; g.tB = 1;
-            g.C = S0c; g.ldc = Np; g.sC = pp;
-            g.alpha = -1.0; g.beta = 1.0; g.batch = nb;
-            gemm(g, st);
+            if (need_S0) {
+                Gemm g;   // S0 -= (B D^{-1}) B^T
+                g.M = Np; g.N = Np; g.K = Nf;
+                g.A = BDc; g.lda = Np; g.sA = pf;
+                g.B = Bd.p; g.ldb = Np; g.sB = pf; g.tB = 1;
+                g.C = S0c; g.ldc = Np; g.sC = pp;
+                g.alpha = -1.0; g.beta = 1.0; g.batch = nb;
+                gemm(g, st);
+            }
         }
         // host copies of the small-r matrices that may need the stability shift
         for (int q = 0; q < nc && n0 + q < s->nstab; ++q) {

@@ -283,6 +283,13 @@ def hartree_from_orbitals(psi_lm: np.ndarray, n_occ: int, lmax_orb: int, lmax_vh
         for q in range(nq):
             acc += Y_rho[:, q][:, None] * dens[:, q][None, :]
         rho_lm[:, sl] = acc
+    return _hartree_from_rho(rho_lm, l_rho, lmax_vh, dr)
+
+
+def _hartree_from_rho(rho_lm: np.ndarray, l_rho: int, lmax_vh: int, dr: float) -> np.ndarray:
+    """V_H[lm, ir] from the density's multipoles rho_lm[lm, ir] (l <= l_rho) by
+    the radial Poisson formula, trapezoid cumulative integrals."""
+    Nr = rho_lm.shape[-1]
     r = np.arange(Nr) * dr
     V = np.zeros(((lmax_vh + 1) ** 2, Nr))
     rp = r[1:]
@@ -398,6 +405,82 @@ def h2o_like(l_cont: int = 20, dr: float = 0.01, r_max: float = 50.0, n_occ: int
     return p
 
 
+def ring_like(l_cont: int = 60, dr: float = 0.02, r_max: float = 100.0, n_occ: int = 60, seed: int = 4,
+              comps: int = 3, l_sce: int = 60) -> Problem:
+    """Planar 16-centre ring: 8 charges +6 on a regular octagon of radius
+    2.7 bohr and 8 charges +9 on radius 5.2 at the same angles (the origin is
+    the centre of charge), n_occ doubly occupied orbitals and one extra-electron
+    initial-state orbital.
+
+    Each orbital's single-centre expansion has `comps` (l, m) components with
+    l <= l_sce (the first orbital's first component at l = l_sce), each radial
+    function a sum of three Slater terms r^l e^{-zeta r} with zeta ~ U(0.8, 8)
+    and N(0, 1) coefficients (seeded).  No two orbitals share a component, so
+    they are orthogonal by their angular parts and only normalised.  V_H comes
+    from the closed-shell density, whose multipoles are Gaunt contractions of
+    the few component pairs per orbital (real Gaunt coefficients by the
+    angular quadrature of degree 2 l_sce + 2 l_cont); the Poisson step is the
+    other configurations'.
+
+    This molecule is too large for the CPU oracle, so nothing compares its
+    inputs across hosts: the density contraction uses BLAS products."""
+    Nr = int(round(r_max / dr)) + 1
+    l_exch = min(l_cont, 10)
+    l_orb = l_cont + l_exch
+    nlm_orb = (l_orb + 1) ** 2
+    n_ch = (l_sce + 1) ** 2
+    rng = np.random.default_rng(seed)
+    n_all = n_occ + 1
+    top = l_sce * l_sce + rng.integers(0, 2 * l_sce + 1)          # an l = l_sce channel
+    rest = rng.permutation(np.setdiff1d(np.arange(n_ch), [top]))[: n_all * comps - 1]
+    chans = np.concatenate([[top], rest]).reshape(n_all, comps)
+    r = np.arange(Nr) * dr
+    r2w = r * r * _simpson_weights(Nr, dr)
+    radial = np.zeros((n_all, comps, Nr))
+    for i in range(n_all):
+        for c in range(comps):
+            l = int(math.isqrt(int(chans[i, c])))
+            zetas = rng.uniform(0.8, 8.0, size=3)
+            coefs = rng.normal(0.0, 1.0, size=3)
+            radial[i, c] = sum(cf * _slater(l, z, dr, Nr) for cf, z in zip(coefs, zetas))
+        nrm = math.sqrt(math.fsum((radial[i] * radial[i] * r2w).ravel()))
+        radial[i] /= nrm
+    F = np.zeros((n_occ, nlm_orb, Nr))
+    for i in range(n_occ):
+        for c in range(comps):
+            F[i, chans[i, c]] = radial[i, c]
+    init = np.zeros((nlm_orb, Nr))
+    for c in range(comps):
+        init[chans[n_occ, c]] = radial[n_occ, c]
+    # density multipoles rho_LM = 2 sum_i sum_{a,b} F_ia F_ib G(a, b, LM)
+    lmax_vh = 2 * l_cont
+    l_rho = min(2 * l_sce, lmax_vh)
+    deg = 2 * l_sce + l_rho
+    Y_a, wq = _ylm_on_grid(l_sce, deg)
+    pairs, cols = [], []
+    for i in range(n_occ):
+        for x in range(comps):
+            for y in range(x, comps):
+                a, b = int(chans[i, x]), int(chans[i, y])
+                pairs.append((i, x, y))
+                cols.append((2.0 if x != y else 1.0) * wq * Y_a[a] * Y_a[b])
+    del Y_a
+    Pm = np.stack(cols, axis=1)                                    # [nq, n_pairs]
+    Y_r, _ = _ylm_on_grid(l_rho, deg)
+    G = Y_r @ Pm                                                   # [n_LM, n_pairs]
+    del Y_r, Pm
+    prod = np.stack([radial[i, x] * radial[i, y] for i, x, y in pairs])   # [n_pairs, Nr]
+    rho_lm = 2.0 * (G @ prod)
+    V_H = _hartree_from_rho(rho_lm, l_rho, lmax_vh, dr)
+    ang = 2.0 * math.pi * np.arange(8) / 8.0
+    xyz = np.concatenate([np.stack([2.7 * np.cos(ang), 2.7 * np.sin(ang), np.zeros(8)], axis=1),
+                          np.stack([5.2 * np.cos(ang), 5.2 * np.sin(ang), np.zeros(8)], axis=1)])
+    p = Problem(name=f"ring_like_l{l_cont}", l_cont=l_cont, n_occ=n_occ, dr=dr, Nr=Nr,
+                Z=np.array([6] * 8 + [9] * 8, dtype=np.int32), xyz=xyz, V_H=V_H, psi_lm=F,
+                orb_energies=np.linspace(-20.0, -0.5, n_occ), init_psi_lm=init, init_energy=-0.25)
+    return p
+
+
 def k_grid(k_min: float, k_max: float, n: int) -> np.ndarray:
     return np.linspace(k_min, k_max, n)
 
@@ -425,4 +508,6 @@ def _build_config(name: str) -> tuple[Problem, np.ndarray]:
         return h2o_like(30, 0.01, 60.0, 5, seed=1, allow_closed=True), k_grid(0.05, 2.0, 256)
     if name == "h2o_l20_pol":    # adjoint / polarisation case
         return h2o_like(20, 0.01, 50.0, 5, seed=1, pol=(9.8, 1.5)), k_grid(0.2, 1.5, 64)
+    if name == "ring_l60":       # large-molecule scale: 16-centre ring, l_cont = 60
+        return ring_like(60, 0.02, 100.0, 60, seed=4), k_grid(0.6, 1.6, 32)
     raise KeyError(name)
