@@ -426,8 +426,10 @@ def main():
                 + ("forward sweep + two recompute/back-propagation passes over the top segments (fit sums, fit "
                    "residual)" if fwd_dip else "three sweeps + one recompute/back-propagation pass over the top "
                                                "segments (fit sums)")
-                + " at the same per-point costs + the per-batch fixed stages (timed on a short-grid scan); a "
-                  "256-energy scan takes 256 / value seconds")
+                + " at the same per-point costs"
+                + (" (per-batch fixed stages not timed: --no-side)" if args.no_side else
+                   " + the per-batch fixed stages (timed on a short-grid scan)")
+                + f"; a {nk}-energy scan takes {nk} / value seconds")
     line = {
         "metric": METRIC,
         "value": value, "unit": "energy points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -1,0 +1,59 @@
+"""The large-molecule configuration's channel space on the device: the
+16-centre ring at l_cont = 60 (N_psi = 3721, N_f = 7260, N_total = 10981:
+0.96 GB per ratio matrix), on a radial grid cut to 401 points so that the
+whole path runs in minutes.  The CPU oracle cannot run this size (one order-
+10981 inverse per radial step), so the checks are what the mathematics and
+the implementation fix:
+* streamed tables (the storage plan's choice at this size) and resident
+  tables give every output bitwise equal, through 64-bit offsets that only
+  this size exercises (0.96 GB matrices, 10981-row Gauss-Jordan leaves);
+* S = (1 + iK)(1 - iK)^{-1} is unitary, K_fit = B A^{-1} is finite and every
+  closed l is fitted as A = 1, B = 0;
+* the integer diagnostics are those of the configuration (n_start from
+  l_exch = 10, the fit window)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from sehf.steppers import GpuContext
+    return GpuContext(True)
+
+
+def _unitarity(K):
+    n = K.shape[0]
+    S = (np.eye(n) + 1j * K) @ np.linalg.inv(np.eye(n) - 1j * K)
+    return float(np.abs(S @ S.conj().T - np.eye(n)).max())
+
+
+@pytest.mark.timeout(1100)
+def test_ring_l60_short_grid(ctx):
+    from sehf import scan, synth
+    p = synth.ring_like(l_cont=60, dr=0.02, r_max=8.0, n_occ=60, seed=4)
+    p.allow_closed = True
+    assert (p.n_psi, p.n_f, p.n_tot, p.Nr) == (3721, 7260, 10981, 401)
+    ks = np.array([1.1])
+    plan = scan.plan_storage(p, len(ks))
+    print(plan["report"])
+    res = {}
+    for tables in ("streamed", "resident"):
+        S = scan.Scan(p, ctx, tables=tables)
+        info = S.info
+        res[tables] = S.run(ks, batch=1, seg_len=plan["seg_len"])
+        print(tables, S.memory())
+        S.close()
+    assert info["n_start"] == 7 and info["n_tot"] == 10981
+    a, b = res["streamed"][0], res["resident"][0]
+    for key in ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D", "diag"):
+        assert np.array_equal(a[key], b[key]), key
+    K = a["K"]
+    assert np.all(np.isfinite(K)) and np.all(np.isfinite(a["D"]))
+    u = _unitarity(K)
+    n_closed = int(a["diag"][10])
+    print(f"ring l60 k=1.1: unitarity {u:.2e}, closed l {n_closed}, |D| {np.abs(a['D']).max():.3e}, "
+          f"diag {a['diag'][4:11]}")
+    assert u < 1e-9
+    assert n_closed > 0           # k r_max = 8.8 < l for most of the 61 l values
