@@ -76,9 +76,14 @@ struct Group {
     Batch bt;
     double t_stage[8] = {0};
     double t_probe = 0.0;   // seconds of the last probe run (autotuner)
+    // SEHF_SCHUR_STREAM=main runs the Schur inverses on the main stream (one
+    // stream per group: the same work in one queue, for tools that follow a
+    // single queue such as per-dispatch counter collection).
     void init() {
         SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        const char* e = std::getenv("SEHF_SCHUR_STREAM");
+        if (e && std::string(e) == "main") st2 = st;
+        else SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_in, &ev_bwd, &ev_s[0], &ev_s[1], &ev_f[0], &ev_f[1]})
             SEHF_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
@@ -93,7 +98,7 @@ struct Group {
         for (hipEvent_t e : {ev_in, ev_bwd, ev_s[0], ev_s[1], ev_f[0], ev_f[1]})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_seg) (void)hipEventDestroy(e);
-        if (st2) (void)hipStreamDestroy(st2);
+        if (st2 && st2 != st) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 };

@@ -1,6 +1,6 @@
 """The large-molecule configuration's channel space on the device: the
 16-centre ring at l_cont = 60 (N_psi = 3721, N_f = 7260, N_total = 10981:
-0.96 GB per ratio matrix), on a radial grid cut to 401 points so that the
+0.96 GB per ratio matrix), on a radial grid cut to 311 points so that the
 whole path runs in minutes.  The CPU oracle cannot run this size (one order-
 10981 inverse per radial step), so the checks are what the mathematics and
 the implementation fix:
@@ -32,9 +32,9 @@ def _unitarity(K):
 @pytest.mark.timeout(1100)
 def test_ring_l60_short_grid(ctx):
     from sehf import scan, synth
-    p = synth.ring_like(l_cont=60, dr=0.02, r_max=8.0, n_occ=60, seed=4)
+    p = synth.ring_like(l_cont=60, dr=0.02, r_max=6.2, n_occ=60, seed=4)
     p.allow_closed = True
-    assert (p.n_psi, p.n_f, p.n_tot, p.Nr) == (3721, 7260, 10981, 401)
+    assert (p.n_psi, p.n_f, p.n_tot, p.Nr) == (3721, 7260, 10981, 311)
     ks = np.array([1.1])
     plan = scan.plan_storage(p, len(ks))
     print(plan["report"])
@@ -56,4 +56,4 @@ def test_ring_l60_short_grid(ctx):
     print(f"ring l60 k=1.1: unitarity {u:.2e}, closed l {n_closed}, |D| {np.abs(a['D']).max():.3e}, "
           f"diag {a['diag'][4:11]}")
     assert u < 1e-9
-    assert n_closed > 0           # k r_max = 8.8 < l for most of the 61 l values
+    assert n_closed > 0           # k r_max = 6.8 < l for most of the 61 l values

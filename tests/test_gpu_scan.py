@@ -138,6 +138,27 @@ def test_scan_groups_bitwise(ctx):
             assert np.array_equal(a[key], c[key]), (key, a["k"])
 
 
+def test_schur_inverses_on_the_main_stream_bitwise(ctx):
+    """SEHF_SCHUR_STREAM=main puts the Schur inverses in the main stream's
+    queue instead of a side stream: the same kernels in one queue, so every
+    output (plain and tangent runs) is bitwise the same."""
+    import os
+    from sehf import scan, synth
+    p = synth.h2o_like(l_cont=4, dr=0.02, r_max=14.0, n_occ=3, seed=1, pol=(9.8, 1.5))
+    ks = np.array([0.5, 1.3])
+    two = scan.Scan(p, ctx)
+    a, at = two.run(ks), two.run(ks, tangent=(1.0, 0.5))
+    os.environ["SEHF_SCHUR_STREAM"] = "main"
+    try:
+        one = scan.Scan(p, ctx)
+        b, bt = one.run(ks), one.run(ks, tangent=(1.0, 0.5))
+    finally:
+        del os.environ["SEHF_SCHUR_STREAM"]
+    for x, y in zip(a + at, b + bt):
+        for key in ("K", "A", "B", "K_fit", "b_overlap", "d_raw", "D") + (("dK", "dD") if "dK" in x else ()):
+            assert np.array_equal(x[key], y[key]), (key, x["k"])
+
+
 def test_scan_polarisation_soft(ctx):
     """The soft polarisation potential -alpha / (2 (r^2 + r_c^2)^2) (alpha = 9.8,
     r_c = 1.5) on the diagonal, small grid: parity with the oracle."""
