@@ -237,4 +237,7 @@ def test_full_size_delay_l20_pol(ctx):
     t_conv = (16 * t2 - t1) / 15
     print(f"  h2o_l20_pol k={k0:.4f}: tau analytic {tau:.9f}, fd(dk) {t1:.9f}, fd(dk/4) {t2:.9f}, "
           f"extrapolated {t_conv:.9f}")
-    assert abs(tau - t_conv) < 1e-5 * max(1.0, abs(t_conv))
+    # the analytic tau within 1e-6 of the converged FD-in-k tau (1.0e-7 measured;
+    # the two FD estimates themselves differ by 5e-5, the O(dk^2) term)
+    assert abs(tau - t_conv) < 1e-6 * max(1.0, abs(t_conv))
+    assert abs(t1 - t_conv) > 10 * abs(tau - t_conv)
