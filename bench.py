@@ -263,7 +263,7 @@ def main():
                     help="energies per batch; 0 = the device storage planner's choice (sehf_plan_storage: the "
                          "widest batch the HBM holds, at most 64); -1 = run the library's autotuner first (timed "
                          "probes, outside the timed region)")
-    ap.add_argument("--tiles", type=int, default=2, help="GEMM tile policy (sehf_set_gemm_tiles); -1 = library default")
+    ap.add_argument("--tiles", type=int, default=0, help="GEMM tile policy (sehf_set_gemm_tiles); -1 = library default")
     ap.add_argument("--config", default="h2o_l30")
     ap.add_argument("--cpu-steps", type=int, default=4, help="radial steps in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")

@@ -259,9 +259,10 @@ int  sehf_scan_run_segment(sehf_scan* s, const double* k, const double* dir, int
 int  sehf_scan_autotune(sehf_scan* s, const double* k, int nk, int probe_steps, int tangent, int* batch_out,
                         int* groups_out, int* tiles_out, double* report, int* n_report);
 /* Process-wide tile policy of the FP64 GEMM: 0 (default) by shape, 128 x 128
- * tiles over 8 waves where M > 64 and K >= 128, 16 x 128 or 32 x 128 over 4
- * waves where M <= 16 or 32, else 64 x 64 over 4 waves; 1 = 64 x 64
- * everywhere; 2 = as 0 but 128 x 128 wherever M > 64.  The tile changes which
+ * tiles over 8 waves where M > 64 and K >= 256, 64 x 128 over 8 waves where
+ * M > 64 and K < 256, 16 x 128 or 32 x 128 over 4 waves where M <= 16 or 32,
+ * else 64 x 64 over 4 waves; 1 = 64 x 64 everywhere; 2 = 128 x 128 wherever
+ * M > 64, else as 0.  The tile changes which
  * lanes accumulate, not the order of the K sum, so results are the same for
  * every policy. */
 int  sehf_set_gemm_tiles(int policy);

@@ -30,9 +30,10 @@ struct Gemm {
 void gemm(const Gemm& g, hipStream_t s);
 
 // GEMM tile policy (process-wide): 0 = by shape (128 x 128 tiles over 8 waves
-// where M > 64 and K >= 128, 16 x 128 or 32 x 128 over 4 waves where M <= 16
-// or 32, else 64 x 64 over 4 waves), 1 = 64 x 64 for every product, 2 = as 0
-// but 128 x 128 wherever M > 64.  With 128 x 128 tiles, a remainder of at
+// where M > 64 and K >= 256, 64 x 128 over 8 waves where M > 64 and K < 256,
+// 16 x 128 or 32 x 128 over 4 waves where M <= 16 or 32, else 64 x 64 over 4
+// waves), 1 = 64 x 64 for every product, 2 = 128 x 128 wherever M > 64, else
+// as 0.  With 128 x 128 tiles, a remainder of at
 // most 64 rows past the last full tile goes to a thin-tile launch.  The
 // autotuner probes the policies; C is bitwise the same under each.
 void set_gemm_tiles(int policy);

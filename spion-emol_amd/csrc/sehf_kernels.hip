@@ -273,8 +273,17 @@ void gemm(const Gemm& g, hipStream_t s) {
     // memory and want the larger grid; 16- and 32-row tiles for the thin
     // block updates of the innermost Gauss-Jordan levels (M = 16 or 32
     // columns), 20-30% faster there than 64-row tiles that are mostly empty.
+    // Policy 0 (default): 128 x 128 for deep products (K >= 256); 64 x 128
+    // over 8 waves (32 x 32 each) for 64 < M with K < 256, the lower levels of
+    // the Gauss-Jordan recursion, where the 128-row tile leaves too few
+    // workgroups (M = 192, K = 192: 229 against 297 us at batch 64).
     const int pol = g_tiles.load(std::memory_order_relaxed);
-    const bool big = g.M > 64 && (pol == 2 || (pol == 0 && g.K >= 128));
+    const bool big = g.M > 64 && (pol == 2 || (pol == 0 && g.K >= 256));
+    if (!big && pol == 0 && g.M > 64) {
+        dim3 grid((g.M + 63) / 64, (g.N + 127) / 128, g.batch);
+        hipLaunchKernelGGL((k_dgemm<64, 128, 2, 4>), grid, dim3(512), 0, s, p);
+        return;
+    }
     if (big) {
         // Rows past the last full 128-row tile: when at most 64 remain, they
         // go to a second launch with a thin tile instead of a mostly empty
