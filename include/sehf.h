@@ -309,6 +309,45 @@ int  sehf_scan_set_groups(sehf_scan* s, int groups);
 #define SEHF_DIPOLES_FORWARD 0
 #define SEHF_DIPOLES_BACKWARD 1
 int  sehf_scan_set_dipole_path(sehf_scan* s, int path);
+
+/* ---- psi files and the dipole-only replay --------------------------------
+ * The reference keeps the back-propagated psi_n of every energy on disk
+ * (main.cpp:1259-1345): the range the dipole integrals read, [0, n_main)
+ * with n_main = min(Nr, max(2, n_transition)), and the 300-point asymptotic
+ * tail [Nr - 300, Nr - 1] that the fit reads (BackPropagator.cpp:27-101,
+ * psi_asym.bin); --dipole-only (main.cpp:167-189, 1278-1320) then redoes only
+ * the fit and the dipoles from them, for instance with another initial
+ * state, and the continuum-continuum dipole codes read them.
+ * sehf_scan_set_psi_store(s, root, keep) makes every later whole scan
+ * (sehf_scan_run, sehf_scan_run_tangent) write, per energy, the directory
+ * <root>/psi_ik<NNNN> (NNNN: the energy's index in that call's k list) with
+ *   psi_asym.bin  (keep & SEHF_PSI_TAIL) the tail, in the reference's
+ *                 single-file layout: 32-byte header 'PSI_ASYM', int32
+ *                 version 1, int32 n_asym = min(300, Nr), int32 first index
+ *                 Nr - n_asym, int32 N_psi, 8 reserved bytes; then n_asym
+ *                 N_psi x N_psi matrices, column-major, ascending n;
+ *   psi_main.bin  (keep & SEHF_PSI_MAIN) [0, n_main) in the same layout with
+ *                 the magic 'PSI_MAIN' (the reference holds this range in
+ *                 its chunked store);
+ *   __SUCCESS__   written last ("k <value>" and the sizes, one per line).
+ * Files are written under temporary names and renamed when complete.  The
+ * tail extends the fit pre-pass down to its first point; SEHF_PSI_MAIN makes
+ * the run back-propagate the dipoles (as SEHF_DIPOLES_BACKWARD).  Each
+ * point costs a device-to-host copy of the batch's psi_n through a pinned
+ * ring and a file write per energy.  root NULL or keep 0 turns it off.
+ * sehf_scan_dipole_only(s, root, k, nk, batch, out) reads the files of
+ * energies 0..nk-1 under root (each directory's recorded k must equal k[i],
+ * N_psi and the tail range must match this problem, psi_main.bin must cover
+ * this problem's dipole range) and runs only the fit and the dipoles, with
+ * this handle's dipole tables (its initial state): A, B, K_fit, d_raw, D,
+ * D_raw, b_overlap, d_corr and diag are written as by sehf_scan_run, bitwise
+ * equal to a back-propagated run of this handle; out->K is left untouched
+ * (K is not formed).  A missing or incomplete directory fails the call. */
+#define SEHF_PSI_TAIL 1
+#define SEHF_PSI_MAIN 2
+int  sehf_scan_set_psi_store(sehf_scan* s, const char* root, int keep);
+int  sehf_scan_dipole_only(sehf_scan* s, const char* root, const double* k, int nk, int batch,
+                           sehf_scan_out* out);
 void sehf_scan_destroy(sehf_scan* s);
 
 /* ---- device storage plan ------------------------------------------------
@@ -327,8 +366,9 @@ void sehf_scan_destroy(sehf_scan* s);
  *         (seg_len x (N_total^2 + N_psi^2) doubles per energy of segment
  *         buffers); the checkpoints live in HBM (SEHF_CKPT_DEVICE) or in
  *         pinned host memory (SEHF_CKPT_HOST), the reference's DISK level;
- *   psi   is never stored (forward-accumulated dipole rows, or integrands
- *         formed during back-propagation).
+ *   psi   is not kept on the device (forward-accumulated dipole rows, or
+ *         integrands formed during back-propagation); it goes to per-energy
+ *         files only when a psi store is set (sehf_scan_set_psi_store).
  * sehf_plan_storage chooses tables, checkpoint level, segment length and
  * batch width: greedily, tables resident while that does not cost batch
  * width, checkpoints on the device while a batch fits, the segment length

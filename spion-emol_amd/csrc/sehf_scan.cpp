@@ -33,6 +33,7 @@
 #include "sehf_common.h"
 #include "sehf_host_math.h"
 #include "sehf_plan.h"
+#include "sehf_psi.h"
 #include "sehf_scan_kernels.h"
 #include "sehf_setup.h"
 #include "sehf_setup_kernels.h"
@@ -76,6 +77,10 @@ struct Group {
     Batch bt;
     double t_stage[8] = {0};
     double t_probe = 0.0;   // seconds of the last probe run (autotuner)
+    // psi files: pinned staging ring between the device and the files
+    static constexpr int kIoSlots = 4;
+    DevArray<double> io_stage;
+    hipEvent_t ev_io[kIoSlots] = {nullptr, nullptr, nullptr, nullptr};
     // SEHF_SCHUR_STREAM=main runs the Schur inverses on the main stream (one
     // stream per group: the same work in one queue, for tools that follow a
     // single queue such as per-dispatch counter collection).
@@ -86,6 +91,7 @@ struct Group {
         else SEHF_HIP_CHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_in, &ev_bwd, &ev_s[0], &ev_s[1], &ev_f[0], &ev_f[1]})
             SEHF_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        for (hipEvent_t& e : ev_io) SEHF_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     void ensure_seg_events(int L) {
         while (static_cast<int>(ev_seg.size()) < L) {
@@ -98,6 +104,8 @@ struct Group {
         for (hipEvent_t e : {ev_in, ev_bwd, ev_s[0], ev_s[1], ev_f[0], ev_f[1]})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_seg) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_io)
+            if (e) (void)hipEventDestroy(e);
         if (st2 && st2 != st) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -134,6 +142,9 @@ struct sehf_scan {
     int dipole_path = SEHF_DIPOLES_FORWARD;    // how the dipole integrals are summed (sehf_scan_set_dipole_path)
     bool sinv_spd = true;                       // Sinv by SPD block elimination (else pivoted Gauss-Jordan)
     int ckpt_where = SEHF_CKPT_DEVICE;          // ratio-matrix checkpoints in HBM or pinned host memory
+    std::string psi_root;                       // per-energy psi files (sehf_scan_set_psi_store), keep bits
+    int psi_keep = 0;
+    std::string replay_root;                    // set while sehf_scan_dipole_only replays from psi files
     std::mutex host_mu;   // lazily computed host tables (stability eigenvalues)
     size_t dev_bytes_shared = 0;
     double t_stage[8] = {0};   // setup, forward, kmatrix, back-prop, fit+dipole, host tables
@@ -536,11 +547,29 @@ void fit_window(int Nr, int& n_fs, int& n_fe) {
 // are back-propagated over the whole grid (and in tangent runs, which always
 // back-propagate), else only the segments from the one holding the fit
 // window's first point (the only back-propagation left).
+// The psi tail, when it is written to file, extends the fit pre-pass down to
+// the tail's first point.
+int psi_tail_lo(int Nr) { return std::max(0, Nr - kPsiTailPoints); }
+
+bool back_propagated_dipoles(const sehf_scan* s, bool tan) {
+    return s->dipole_path == SEHF_DIPOLES_BACKWARD || tan || (s->psi_keep & SEHF_PSI_MAIN);
+}
+
 int ckpt_lo(const sehf_scan* s, int L, bool tan) {
-    if (s->dipole_path == SEHF_DIPOLES_BACKWARD || tan) return 0;
+    if (back_propagated_dipoles(s, tan)) return 0;
     int n_fs = 0, n_fe = 0;
     fit_window(s->Nr, n_fs, n_fe);
+    if (s->psi_keep & SEHF_PSI_TAIL) n_fs = std::min(n_fs, psi_tail_lo(s->Nr));
     return n_fs / L;
+}
+
+PsiLayout psi_layout(const sehf_scan* s) {
+    PsiLayout l;
+    l.Np = s->Np; l.Nr = s->Nr;
+    l.tail_lo = psi_tail_lo(s->Nr);
+    l.n_tail = s->Nr - l.tail_lo;
+    l.n_main = s->n_pts;
+    return l;
 }
 
 // One batch of energies through the whole path.  With `dir` (per energy the
@@ -591,6 +620,50 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // (= (W^{-1})[:, psi]^T) for tangent runs, else its own buffer
     double* const Xd = tan ? b.Gt.p + pp : b.X.p;
     const int64_t sXd = tan ? static_cast<int64_t>(Np) * Nt : static_cast<int64_t>(pf);
+    // psi files: written by a whole-scan run with a psi store set; read back
+    // (instead of solving) while sehf_scan_dipole_only replays the fit and
+    // the dipoles
+    const bool replay = !s->replay_root.empty();
+    const PsiLayout plo = psi_layout(s);
+    std::unique_ptr<PsiWriter> pw;
+    std::unique_ptr<PsiReader> pr;
+    if (replay) pr.reset(new PsiReader(s->replay_root, out_off, nb, ks, plo, s->n_pts > 0));
+    else if (s->psi_keep && seg < 0 && probe == 0)
+        pw.reset(new PsiWriter(s->psi_root, out_off, nb, ks, plo, s->psi_keep & SEHF_PSI_TAIL, s->psi_keep & SEHF_PSI_MAIN));
+    if ((pw || pr) && g.io_stage.n < static_cast<size_t>(Group::kIoSlots) * nb * pp)
+        g.io_stage.alloc_host(static_cast<size_t>(Group::kIoSlots) * nb * pp);
+    // Ring of kIoSlots pinned slots: a slot's file write (or the copy that
+    // read it) is completed just before the slot is used again, so the host
+    // keeps a few points of device work queued while it does file I/O.
+    int io_next = 0;
+    int io_pend[Group::kIoSlots][2];
+    for (auto& q : io_pend) { q[0] = -1; q[1] = -1; }
+    auto io_slot = [&](int i) { return g.io_stage.p + static_cast<size_t>(i) * nb * pp; };
+    auto io_retire = [&](int i) {
+        if (io_pend[i][0] < 0) return;
+        SEHF_HIP_CHECK(hipEventSynchronize(g.ev_io[i]));
+        if (pw) pw->put(io_pend[i][0], io_pend[i][1], io_slot(i));
+        io_pend[i][0] = -1;
+    };
+    auto psi_out = [&](int file, int n, const double* dpsi) {
+        if (!pw || file < 0 || !pw->wants(file, n)) return;
+        const int i = io_next++ % Group::kIoSlots;
+        io_retire(i);
+        SEHF_HIP_CHECK(hipMemcpyAsync(io_slot(i), dpsi, nb * pp * sizeof(double), hipMemcpyDeviceToHost, g.st));
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_io[i], g.st));
+        io_pend[i][0] = file; io_pend[i][1] = n;
+    };
+    auto psi_in = [&](int n, double* dpsi) {
+        const int i = io_next++ % Group::kIoSlots;
+        io_retire(i);
+        pr->get(n, io_slot(i));
+        SEHF_HIP_CHECK(hipMemcpyAsync(dpsi, io_slot(i), nb * pp * sizeof(double), hipMemcpyHostToDevice, g.st));
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_io[i], g.st));
+        io_pend[i][0] = 2; io_pend[i][1] = n;
+    };
+    auto io_drain = [&] {
+        for (int i = 0; i < Group::kIoSlots; ++i) io_retire(i);
+    };
     // ---------------- host tables for this batch ----------------
     std::vector<double> E(nb), cE(nb);
     for (int e = 0; e < nb; ++e) { E[e] = 0.5 * ks[e] * ks[e]; cE[e] = s->c6 * E[e]; }
@@ -920,7 +993,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // back-propagated dZ_n = Rinv_n (dZ_{n+1} + H_n Z_n) carries it once.
     // Below n_start the ratio matrices are the analytic diagonal; the dipole
     // rows there still need those points' Sinv_n, so the loop starts at n = 1.
-    const bool adj = s->dipole_path == SEHF_DIPOLES_FORWARD && !tan;
+    const bool adj = !back_propagated_dipoles(s, tan) && !replay;
     const int n_lo = (adj && seg < 0 && probe == 0) ? std::min(1, n_first) : n_first;
     // B_n D_n^{-1} of the forward sweep: resident, or the ring slot of point n
     auto bd_fwd = [&](int n) -> double* {
@@ -932,8 +1005,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb, bd_fwd(n));
         SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
     };
-    const int n_last = probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
-                                 : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
+    const int n_last = replay ? -1   // a replay solves nothing
+                              : probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
+                                          : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
     const int64_t sAd = static_cast<int64_t>(ns) * Nt;
     auto gm = [&](double* C, int64_t ldc, int64_t sC, const double* A, int64_t lda, int64_t sA, const double* Bm,
                   int64_t ldb, int64_t sB, int M, int Nn, int K, double beta) {
@@ -1007,7 +1081,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     const double tw2 = wall();
     if (seg_t) { seg_t[0] = tw1 - tw0; seg_t[1] = tw2 - tw1; }
     // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
-    if (seg < 0) {
+    if (seg < 0 && !replay) {
     SEHF_HIP_CHECK(hipMemcpyAsync(b.Rc.p, b.Rp.p, nb * tt * sizeof(double), hipMemcpyDeviceToDevice, st));
     gj(b.Rc.p, b.U.p, Nt, false);   // U <- R = Rinv_{N-1}^{-1}
     get_block(b.Am.p, b.U.p, Nt, tt, 0, 0, Np, Np, nb, st);   // S_schur starts as R_psi_psi
@@ -1078,6 +1152,8 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     }
     // ---------------- outer boundary (BackPropagator.cpp:128-197, :538-551) ----------------
     psi_boundary(b.P.p, b.Kd.p, b.jy.p, Np, nb, st);
+    psi_out(0, N, b.P.p);
+    psi_out(1, N, b.P.p);
     }   // seg < 0
     // Z_psi = psi + h^2/6 (E psi - V_N psi) + B_N f_N with f_N = -D_N^{-1} B_N^T psi, Z_f = 0
     auto zmap = [&](const double* Pin, double* Zout) {
@@ -1103,7 +1179,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         }
         z_boundary(Zout, Pin, b.Vpsi.p, b.Bf.p, b.cE.p, s->c6, Np, Nt, nb, st);
     };
-    if (seg < 0) zmap(b.P.p, b.Z.p);
+    if (seg < 0 && !replay) zmap(b.P.p, b.Z.p);
     if (tan && seg < 0) {
         // d psi_N from dK and the tables; dZ = zmap(d psi_N) + h^2/6 w_N psi_N
         dpsi_boundary(b.dP.p, b.Kd.p, b.dKd.p, b.jy.p, b.djy.p, Np, nb, st);
@@ -1126,7 +1202,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         const size_t o = static_cast<size_t>(n - n_fs) * Np;
         fit_accum_t(b.dYJ.p, b.dYY.p, psi, dpsi, b.Jc.p + o, b.Yc.p + o, b.dJc.p + o, b.dYc.p + o, sJc, Np, nb, st);
     };
-    if (N < s->n_pts && seg < 0) dip_accum(N, b.P.p, b.acc.p);
+    if (N < s->n_pts && seg < 0) {
+        if (replay) psi_in(N, b.P.p);
+        dip_accum(N, b.P.p, b.acc.p);
+    }
     // The fit window [n_fs, n_fe] ends 3 points below the boundary N.  Its
     // sums YJ, YY come from a pre-pass over the top segments (below), so the
     // main pass knows A and B and takes the fit residual point by point; no
@@ -1191,6 +1270,8 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // A segment cycle runs a Full pass over its one segment (sg_hi = sg_lo)
     // and, with t_rec set, records when the segment's recompute has finished.
     enum class Pass { FitSums, Resid, Full };
+    // the fit pre-pass also covers the psi tail when that is written to file
+    const int fs_lo = (pw && pw->wants(0, plo.tail_lo)) ? std::min(n_fs, plo.tail_lo) : n_fs;
     bool seg_pending = false;   // a previous segment's backward products may still read segS
     double t_rec = 0.0;
     auto sweep = [&](Pass kind, int sg_hi, int sg_lo, bool time_rec) {
@@ -1252,13 +1333,14 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             progress(main ? "back-propagation segment" : kind == Pass::FitSums ? "fit pre-pass segment"
                                                                                 : "fit residual segment",
                      sg, n_seg);
-            const int n_stop = main ? n0 : std::max(n0, n_fs);
+            const int n_stop = main ? n0 : std::max(n0, kind == Pass::FitSums ? fs_lo : n_fs);
             for (int n = n1 - 1; n >= n_stop; --n) {
                 const double* Rn = b.segR.p + static_cast<size_t>(n - n0) * nb * tt;
                 const double* Sn = b.segS.p + static_cast<size_t>(n - n0) * nb * pp;
                 mm(b.Zt.p, Rn, b.Z.p, Nt, Np, Nt, Nt, tt, Nt, sZ, Nt, sZ, 0.0);   // Z <- Rinv_n Z
                 std::swap(b.Z.p, b.Zt.p);
                 winv_psi(n, b.Z.p, Sn, b.Psi.p, bd_seg(n));                          // psi_n
+                psi_out(kind == Pass::FitSums ? 0 : main ? 1 : -1, n, b.Psi.p);
                 const bool in_win = n >= n_fs && n <= n_fe;
                 if (main && n < s->n_pts) dip_accum(n, b.Psi.p, b.acc.p);
                 if (in_win) {
@@ -1335,13 +1417,33 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (seg_t) { seg_t[2] = t_rec - tb0; seg_t[3] = tb1 - t_rec; }
         return;
     }
-    sweep(Pass::FitSums, n_seg - 1, n_fs / L, false);
-    // A, B of the fit from the completed window sums; the next pass starts
-    // again from the outer boundary
-    fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
-    zmap(b.P.p, b.Z.p);
-    if (adj) sweep(Pass::Resid, n_seg - 1, n_fs / L, false);
-    else sweep(Pass::Full, n_seg - 1, 0, false);
+    if (replay) {
+        // the passes of a back-propagated scan, in the same order over the
+        // same psi_n, read from the files: window sums, A and B, then the
+        // dipole sums and the fit residual from the top down
+        for (int n = n_fe; n >= n_fs; --n) {
+            psi_in(n, b.Psi.p);
+            fit_sums(n, b.Psi.p);
+        }
+        fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
+        for (int n = N - 1; n >= 0; --n) {
+            const bool in_win = n >= n_fs && n <= n_fe, in_dip = n < s->n_pts;
+            if (!in_win && !in_dip) continue;
+            psi_in(n, b.Psi.p);
+            if (in_dip) dip_accum(n, b.Psi.p, b.acc.p);
+            if (in_win) fit_resid_at(n, b.Psi.p);
+            progress("dipole replay n =", n, N);
+        }
+    } else {
+        sweep(Pass::FitSums, n_seg - 1, fs_lo / L, false);
+        // A, B of the fit from the completed window sums; the next pass starts
+        // again from the outer boundary
+        fit_solve(b.Am.p, b.Bm.p, b.YJ.p, b.YY.p, b.coef.p, Np, nb, st);
+        zmap(b.P.p, b.Z.p);
+        if (adj) sweep(Pass::Resid, n_seg - 1, n_fs / L, false);
+        else sweep(Pass::Full, n_seg - 1, 0, false);
+    }
+    io_drain();
     check_gj_info(b.gj, nb, st, "scan back-propagation");
     check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
@@ -1410,7 +1512,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (dst) SEHF_HIP_CHECK(hipMemcpyAsync(dst + static_cast<size_t>(out_off) * per, src, per * nb * sizeof(double),
                                                hipMemcpyDeviceToHost, st));
     };
-    dl(out->K, b.Kd.p, pp);
+    if (!replay) dl(out->K, b.Kd.p, pp);   // a replay does not form K
     dl(out->A, b.Am.p, pp);
     dl(out->B, b.Bm.p, pp);
     dl(out->K_fit, b.Kfit.p, pp);
@@ -1487,6 +1589,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             d[8] = shA[e]; d[9] = shS[e]; d[10] = n_closed[e];
         }
     }
+    if (pw) pw->commit();
 }
 
 }  // namespace
@@ -1584,8 +1687,11 @@ void scan_run(sehf_scan* s, const double* k, const double* dir, int nk, int batc
         s->groups.emplace_back(new Group());
         s->groups.back()->init();
     }
-    for (int gi = 0; gi < G; ++gi) alloc_batch(s, *s->groups[gi], B, L, n_seg - ckpt_lo(s, L, dir != nullptr), n_fit,
-                                                 dir != nullptr);
+    // a replay from psi files needs no segment buffers or checkpoints
+    const bool replay = !s->replay_root.empty();
+    for (int gi = 0; gi < G; ++gi)
+        alloc_batch(s, *s->groups[gi], B, replay ? 1 : L, replay ? 1 : n_seg - ckpt_lo(s, L, dir != nullptr), n_fit,
+                    dir != nullptr);
     if (out->d_corr) std::memcpy(out->d_corr, s->dcorr_h.data(), s->dcorr_h.size() * sizeof(double));
     // batch i runs on group i % G; each group is driven by its own host thread
     std::vector<std::string> errs(G);
@@ -1797,6 +1903,34 @@ int sehf_scan_set_groups(sehf_scan* s, int groups) {
         if (groups < 1 || groups > 4)
             throw std::runtime_error("sehf_scan_set_groups: groups must be in [1, 4] (one stream each)");
         s->n_groups = groups;
+    });
+}
+
+int sehf_scan_set_psi_store(sehf_scan* s, const char* root, int keep) {
+    return guarded([&] {
+        if (!s) throw std::runtime_error("sehf_scan_set_psi_store: NULL scan");
+        if (keep & ~(SEHF_PSI_TAIL | SEHF_PSI_MAIN))
+            throw std::runtime_error("sehf_scan_set_psi_store: keep must be a combination of SEHF_PSI_TAIL and SEHF_PSI_MAIN");
+        if (!root || !*root || keep == 0) {
+            s->psi_root.clear();
+            s->psi_keep = 0;
+            return;
+        }
+        s->psi_root = root;
+        s->psi_keep = keep;
+    });
+}
+
+int sehf_scan_dipole_only(sehf_scan* s, const char* root, const double* k, int nk, int batch, sehf_scan_out* out) {
+    return guarded([&] {
+        if (!s) throw std::runtime_error("sehf_scan_dipole_only: NULL scan");
+        if (!root || !*root) throw std::runtime_error("sehf_scan_dipole_only: root must name the psi store");
+        struct Clear {
+            sehf_scan* s;
+            ~Clear() { s->replay_root.clear(); }
+        } clear{s};
+        s->replay_root = root;
+        scan_run(s, k, nullptr, nk, batch, 0, out, nullptr);
     });
 }
 

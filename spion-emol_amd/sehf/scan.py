@@ -8,6 +8,7 @@ with the DipoleEnergyPayload fields (src/scatt/DipoleIO.hpp:82-112).
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -99,6 +100,11 @@ def _register(L):
     L.sehf_scan_memory.restype = ctypes.c_int
     L.sehf_plan_storage.argtypes = [ctypes.POINTER(PlanIn_C), ctypes.POINTER(Plan_C)]
     L.sehf_plan_storage.restype = ctypes.c_int
+    L.sehf_scan_set_psi_store.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+    L.sehf_scan_set_psi_store.restype = ctypes.c_int
+    L.sehf_scan_dipole_only.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ScanOut_C)]
+    L.sehf_scan_dipole_only.restype = ctypes.c_int
     L._scan_registered = True
     return L
 
@@ -156,12 +162,7 @@ class Scan:
             check(L.sehf_scan_set_groups(self._h, int(groups)), "sehf_scan_set_groups")
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         nk, N, no = len(ks), self.p.n_psi, self.p.n_occ
-        o = {"K": np.zeros((nk, N, N)), "A": np.zeros((nk, N, N)), "B": np.zeros((nk, N, N)),
-             "K_fit": np.zeros((nk, N, N)), "b_overlap": np.zeros((nk, no, N)), "d_raw": np.zeros((nk, 6, N)),
-             "D_raw": np.zeros((nk, 6, N), dtype=np.complex128), "D": np.zeros((nk, 6, N), dtype=np.complex128),
-             "d_corr": np.zeros((6, no)), "diag": np.zeros((nk, 16))}
-        out = ScanOut_C(*[o[n].ctypes.data_as(_dp) for n in ("K", "A", "B", "K_fit", "b_overlap", "d_raw",
-                                                             "D_raw", "D", "d_corr", "diag")])
+        o, out = self._outputs(nk)
         if tangent is None:
             check(L.sehf_scan_run(self._h, ks.ctypes.data_as(_dp), nk, batch, seg_len, ctypes.byref(out)),
                   "sehf_scan_run")
@@ -176,18 +177,55 @@ class Scan:
             tout = TangentOut_C(*[o[n].ctypes.data_as(_dp) for n in ("dK", "dA", "dB", "db_overlap", "dd_raw", "dD")])
             check(L.sehf_scan_run_tangent(self._h, ks.ctypes.data_as(_dp), d.ctypes.data_as(_dp), nk, batch,
                                           seg_len, ctypes.byref(out), ctypes.byref(tout)), "sehf_scan_run_tangent")
+        return self._results(o, ks, tangent is not None)
+
+    def _outputs(self, nk):
+        N, no = self.p.n_psi, self.p.n_occ
+        o = {"K": np.zeros((nk, N, N)), "A": np.zeros((nk, N, N)), "B": np.zeros((nk, N, N)),
+             "K_fit": np.zeros((nk, N, N)), "b_overlap": np.zeros((nk, no, N)), "d_raw": np.zeros((nk, 6, N)),
+             "D_raw": np.zeros((nk, 6, N), dtype=np.complex128), "D": np.zeros((nk, 6, N), dtype=np.complex128),
+             "d_corr": np.zeros((6, no)), "diag": np.zeros((nk, 16))}
+        out = ScanOut_C(*[o[n].ctypes.data_as(_dp) for n in ("K", "A", "B", "K_fit", "b_overlap", "d_raw",
+                                                             "D_raw", "D", "d_corr", "diag")])
+        return o, out
+
+    @staticmethod
+    def _results(o, ks, tangent):
         res = []
-        for e in range(nk):
+        for e in range(len(ks)):
             # matrices come back column-major per energy
             r = {"k": float(ks[e]), "K": o["K"][e].T.copy(), "A": o["A"][e].T.copy(),
                  "B": o["B"][e].T.copy(), "K_fit": o["K_fit"][e].T.copy(),
                  "b_overlap": o["b_overlap"][e].T.copy(), "d_raw": o["d_raw"][e], "D_raw": o["D_raw"][e],
                  "D": o["D"][e], "d_corr": o["d_corr"], "diag": o["diag"][e]}
-            if tangent is not None:
+            if tangent:
                 r.update({"dK": o["dK"][e].T.copy(), "dA": o["dA"][e].T.copy(), "dB": o["dB"][e].T.copy(),
                           "db_overlap": o["db_overlap"][e].T.copy(), "dd_raw": o["dd_raw"][e], "dD": o["dD"][e]})
             res.append(r)
         return res
+
+    PSI_TAIL = 1   # SEHF_PSI_TAIL
+    PSI_MAIN = 2   # SEHF_PSI_MAIN
+
+    def set_psi_store(self, root, keep: int = 3):
+        """Later whole scans write per energy <root>/psi_ik<NNNN>/ with the
+        300-point asymptotic tail of psi (psi_asym.bin, keep & PSI_TAIL) and
+        the dipole range [0, n_pts) (psi_main.bin, keep & PSI_MAIN); None
+        turns it off (sehf_scan_set_psi_store).  `sehf.psi_files` reads them."""
+        r = None if root is None else os.fsencode(os.fspath(root))
+        check(load().sehf_scan_set_psi_store(self._h, r, int(keep) if r is not None else 0),
+              "sehf_scan_set_psi_store")
+
+    def dipole_only(self, ks, root, batch: int = 0):
+        """The fit and the dipoles of energies ks from the psi files under
+        root (written by a scan with set_psi_store over the same k list), with
+        this handle's dipole tables; "K" is not formed and stays zero
+        (sehf_scan_dipole_only)."""
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        o, out = self._outputs(len(ks))
+        check(load().sehf_scan_dipole_only(self._h, os.fsencode(os.fspath(root)), ks.ctypes.data_as(_dp), len(ks),
+                                           int(batch), ctypes.byref(out)), "sehf_scan_dipole_only")
+        return self._results(o, ks, False)
 
     def set_checkpoints(self, where: str):
         """Ratio-matrix checkpoints in "device" memory or pinned "host"
