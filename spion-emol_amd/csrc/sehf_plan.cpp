@@ -106,6 +106,7 @@ extern "C" int sehf_plan_storage(const sehf_plan_in* in, sehf_plan* out) {
         BatchDims d;
         d.Np = Np; d.Nf = Nf; d.Nt = Nt; d.ns = ns; d.n_occ = q.n_occ;
         d.nstab = std::min<size_t>(60, Nr); d.n_start = 1; d.Nr = Nr;
+        d.sinv_group = sinv_group_for(Np, true);   // the scan's default Schur inverse (SPD block elimination)
         const bool tan = q.tangent != 0;
         const bool fwd = q.dipole_path == SEHF_DIPOLES_FORWARD && !tan;
         const int nfs = fit_lo(q.Nr);

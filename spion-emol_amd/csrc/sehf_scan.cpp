@@ -51,7 +51,7 @@ struct Batch {
     DevArray<double> Rp, Rc, U, Ms, X, Z, Zt, T, Psi, acc, YJ, YY, Am, Bm, Kd, P, Vpsi, G, Bf;
     DevArray<double> ckpt, segR, segS, W2, W2i, Rff, Rffi, T1, DRre, DRim, ore, oim, Dre, Dim, Ainv, Kfit;
     DevArray<double> dA, dS, zeros, cE, anl, tab, jy, Jc, Yc, coef, red, nrm;
-    DevArray<double> SinvR;   // forward sweep: Sinv ring of two, filled one step ahead on the side stream
+    DevArray<double> SinvR;   // forward sweep: Sinv ring of two groups of points, filled a group ahead on the side stream
     DevArray<double> Ad, Ad2;   // forward-accumulated dipole rows (ns x Nt per energy, ping-pong)
     // streamed tables (energy independent, shared by the batch): S0_n, V_n and
     // B_n scratch of the side stream, B D^{-1} of the two forward ring slots
@@ -451,11 +451,11 @@ void batch_arrays(Batch& b, const BatchDims& d, int B, int L, int n_ck, int n_fi
     al(b.tab, 6 * Np * B, false); al(b.jy, 2 * Np * B, false);
     al(b.Jc, std::max<size_t>(n_fit, 1) * Np * B, false); al(b.Yc, std::max<size_t>(n_fit, 1) * Np * B, false);
     al(b.coef, 5 * Np * B, false); al(b.red, 2 * static_cast<size_t>(B), false); al(b.nrm, 2 * static_cast<size_t>(B), false);
-    al(b.SinvR, 2 * pp * B, false);
+    al(b.SinvR, 2 * d.sinv_group * pp * B, false);
     al(b.Ad, ns * Nt * B, false); al(b.Ad2, ns * Nt * B, false);
     if (d.streamed) {
         al(b.tS0, pp, false); al(b.tV, pp, false); al(b.tB, pf, false);
-        al(b.ringBD, 2 * pf, false); al(b.segBD, pf * L, false);
+        al(b.ringBD, 2 * d.sinv_group * pf, false); al(b.segBD, pf * L, false);
     }
     if (tan) {
         const size_t no = d.n_occ, nx = std::max<size_t>(6, no);
@@ -484,7 +484,7 @@ BatchBytes batch_bytes(const BatchDims& d, int B, int L, int n_ck, int n_fit, bo
     batch_arrays(b, d, B, L, n_ck, n_fit, tan, [&](auto& arr, size_t count, bool host) {
         (host ? out.host : out.device) += count * sizeof(*arr.p);
     });
-    out.device += gj_buffer_bytes(std::max(d.Nt, 2 * d.Np), B) + gj_buffer_bytes(d.Np, B);
+    out.device += gj_buffer_bytes(std::max(d.Nt, 2 * d.Np), B) + gj_buffer_bytes(d.Np, B * d.sinv_group);
     return out;
 }
 
@@ -498,6 +498,7 @@ BatchDims dims_of(const sehf_scan* s) {
     d.nstab = s->nstab; d.n_start = s->n_start; d.Nr = s->Nr;
     d.streamed = s->tables == SEHF_TABLES_STREAMED;
     d.ckpt_host = s->ckpt_where == SEHF_CKPT_HOST;
+    d.sinv_group = sinv_group_for(d.Np, s->sinv_spd);
     return d;
 }
 
@@ -518,7 +519,7 @@ void alloc_batch(sehf_scan* s, Group& g, int B, int L, int n_ck, int n_fit, bool
     });
     fill(b.zeros.p, B, 0.0, g.st);
     b.gj.ensure(static_cast<int>(std::max(d.Nt, 2 * d.Np)), B);
-    b.gj2.ensure(static_cast<int>(d.Np), B);
+    b.gj2.ensure(static_cast<int>(d.Np), B * static_cast<int>(d.sinv_group));
     b.has_tan = tan;
     g.ensure_seg_events(L);
 }
@@ -616,6 +617,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     const int n_seg = (N + L - 1) / L;
     const double h = s->h;
     const bool tan = dir != nullptr;
+    // radial points per Schur-inverse call (the SPD inverses of sgrp points
+    // share one launch sequence; one point with the Gauss-Jordan inverse)
+    const int sgrp = sinv_group_for(static_cast<size_t>(s->Np), s->sinv_spd);
     // -Sinv (B D^{-1}) of the current step: inside Gt = [Sinv | -Sinv B D^{-1}]
     // (= (W^{-1})[:, psi]^T) for tangent runs, else its own buffer
     double* const Xd = tan ? b.Gt.p + pp : b.X.p;
@@ -869,7 +873,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // zero-pivot flags start clear for this batch (a failed earlier run on
     // the same buffers may have left them set)
     fill_int(b.gj.info.p, static_cast<size_t>(nb), 0, st);
-    fill_int(b.gj2.info.p, static_cast<size_t>(nb), 0, st2);
+    fill_int(b.gj2.info.p, static_cast<size_t>(nb) * sgrp, 0, st2);
     const int64_t sJc = static_cast<int64_t>(n_fit) * Np;
 
     auto gj = [&](double* mat, double* outp, int n, bool sym) {
@@ -884,8 +888,10 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     // selects the pivoted Gauss-Jordan inverse instead.
     // Streamed tables: the side stream rebuilds S0_n (and B_n D_n^{-1} into
     // bd_out, where the consumers of point n read it) before the Schur inverse.
+    // A group of sgrp consecutive points is built point by point (sinv_build)
+    // into consecutive slots and then inverted by one call (sinv_invert).
     const bool streamed = s->tables == SEHF_TABLES_STREAMED;
-    auto sinv_step = [&](int n, double* Sout, double* bd_out) {
+    auto sinv_build = [&](int n, double* Sout, double* bd_out) {
         const double* a = (n < s->nstab) ? b.dA.p + static_cast<size_t>(n) * nb : b.zeros.p;
         const double* d = (n < s->nstab) ? b.dS.p + static_cast<size_t>(n) * nb : b.zeros.p;
         double* Sm = s->sinv_spd ? Sout : b.Ms.p;
@@ -896,8 +902,11 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         }
         sinv_matrix(Sm, S0n, s->cvd.p + static_cast<size_t>(n) * Np,
                     s->bdb.p + static_cast<size_t>(n) * Np, b.cE.p, a, d, Np, nb, st2);
-        if (s->sinv_spd) spd_inverse(Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
-        else gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
+        if (!s->sinv_spd) gj_inverse(b.Ms.p, Sout, Np, Np, static_cast<int64_t>(pp), nb, true, b.gj2.work(), st2);
+    };
+    // cnt consecutive points' matrices from S (cnt x nb, stride N_psi^2)
+    auto sinv_invert = [&](double* S, int cnt) {
+        if (s->sinv_spd) spd_inverse(S, Np, Np, static_cast<int64_t>(pp), nb * cnt, true, b.gj2.work(), st2);
     };
     auto frp_step = [&](int n, const double* Sv, const double* Rprev, double* Rout, const double* BDn) {
         init_U(b.U.p, Rprev, Sv, s->Dinv.p + static_cast<size_t>(n) * std::max(Nf, 1), Np, Nt, nb, st);
@@ -968,9 +977,9 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         if (ns0 >= 2) diag_matrix(b.dRp.p, b.danl.p + static_cast<size_t>(ns0 - 1) * nb * Nt, Nt, nb, st);
         else fill(b.dRp.p, tt * nb, 0.0, st);
     }
-    // Sinv_n goes to ring slot n & 1, one step ahead of its consumer: the
-    // side stream waits until FRP(n-2) has read the slot; FRP(n) waits for
-    // Sinv_n.
+    // The Schur inverses go to a ring of two halves of sgrp points each: the
+    // side stream inverts group q + 1 while the main stream runs the ratio
+    // steps of group q; FRP(n) waits for its group's inverses.
     // A segment cycle's forward part starts at the segment's first point from
     // the analytic start matrix (a ratio matrix of the right size and
     // conditioning; the cycle's results are not kept).
@@ -996,18 +1005,28 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     const bool adj = !back_propagated_dipoles(s, tan) && !replay;
     const int n_lo = (adj && seg < 0 && probe == 0) ? std::min(1, n_first) : n_first;
     // B_n D_n^{-1} of the forward sweep: resident, or the ring slot of point n
-    auto bd_fwd = [&](int n) -> double* {
-        return streamed ? b.ringBD.p + static_cast<size_t>(n & 1) * pf : s->BD.p + static_cast<size_t>(n) * pf;
-    };
-    auto sinv_ahead = [&](int n) {
-        const int slot = n & 1;
-        if (n - 2 >= n_lo) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[slot], 0));
-        sinv_step(n, b.SinvR.p + static_cast<size_t>(slot) * pp * nb, bd_fwd(n));
-        SEHF_HIP_CHECK(hipEventRecord(g.ev_s[slot], st2));
-    };
     const int n_last = replay ? -1   // a replay solves nothing
                               : probe > 0 ? std::min(Nr - 2, ns0 + probe - 1)
                                           : (seg >= 0 ? std::min(seg * L + L, N) - 1 : Nr - 2);
+    // point n is entry (n - n_lo) % sgrp of group q = (n - n_lo) / sgrp, in
+    // ring half q & 1
+    auto ring_idx = [&](int n) {
+        const int q = (n - n_lo) / sgrp;
+        return static_cast<size_t>((q & 1) * sgrp + (n - n_lo) % sgrp);
+    };
+    auto bd_fwd = [&](int n) -> double* {
+        return streamed ? b.ringBD.p + ring_idx(n) * pf : s->BD.p + static_cast<size_t>(n) * pf;
+    };
+    auto sinv_slot = [&](int n) { return b.SinvR.p + ring_idx(n) * pp * nb; };
+    // group q's Schur inverses, after the ratio steps of group q - 2 (the
+    // previous users of its ring half) are done
+    auto sinv_ahead = [&](int q) {
+        const int a = n_lo + q * sgrp, e = std::min(a + sgrp, n_last + 1);
+        if (q >= 2) SEHF_HIP_CHECK(hipStreamWaitEvent(st2, g.ev_f[q & 1], 0));
+        for (int n = a; n < e; ++n) sinv_build(n, sinv_slot(n), bd_fwd(n));
+        sinv_invert(sinv_slot(a), e - a);
+        SEHF_HIP_CHECK(hipEventRecord(g.ev_s[q & 1], st2));
+    };
     const int64_t sAd = static_cast<int64_t>(ns) * Nt;
     auto gm = [&](double* C, int64_t ldc, int64_t sC, const double* A, int64_t lda, int64_t sA, const double* Bm,
                   int64_t ldb, int64_t sB, int M, int Nn, int K, double beta) {
@@ -1029,12 +1048,15 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         std::swap(b.Ad.p, b.Ad2.p);
     };
     if (adj) fill(b.Ad.p, static_cast<size_t>(sAd) * nb, 0.0, st);
-    if (n_lo <= n_last) sinv_ahead(n_lo);
+    if (n_lo <= n_last) sinv_ahead(0);
     for (int n = n_lo; n <= n_last; ++n) {
-        if (n + 1 <= n_last) sinv_ahead(n + 1);
-        const int slot = n & 1;
-        SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
-        const double* Sslot = b.SinvR.p + static_cast<size_t>(slot) * pp * nb;
+        const int q = (n - n_lo) / sgrp, slot = q & 1;
+        const bool first = (n - n_lo) % sgrp == 0, last = (n - n_lo) % sgrp == sgrp - 1 || n == n_last;
+        if (first) {
+            if (n_lo + (q + 1) * sgrp <= n_last) sinv_ahead(q + 1);
+            SEHF_HIP_CHECK(hipStreamWaitEvent(st, g.ev_s[slot], 0));
+        }
+        const double* Sslot = sinv_slot(n);
         if (n < ns0) {
             // analytic region (forward-accumulated dipoles only): Rinv_n =
             // diag(anl_n) built in Rc; Rp holds the start matrix Rinv_{n_start-1}
@@ -1048,13 +1070,13 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             }
             diag_matrix(b.Rc.p, b.anl.p + static_cast<size_t>(n) * nb * Nt, Nt, nb, st);
             adj_step(n, Sslot, b.Rc.p);
-            SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
+            if (last) SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
             continue;
         }
         frp_step(n, Sslot, b.Rp.p, b.Rc.p, bd_fwd(n));
         if (adj) adj_step(n, Sslot, b.Rc.p);
         if (tan) frp_tangent(n, Sslot, b.Rc.p, b.dRp.p, b.dRc.p);
-        SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
+        if (last) SEHF_HIP_CHECK(hipEventRecord(g.ev_f[slot], st));
         std::swap(b.Rp.p, b.Rc.p);
         if (tan) std::swap(b.dRp.p, b.dRc.p);
         progress("forward sweep n =", n, Nr - 2);
@@ -1077,7 +1099,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         return;
     }
     check_gj_info(b.gj, nb, st, "scan forward sweep");
-    check_gj_info(b.gj2, nb, st2, "scan forward sweep (Schur inverses)");
+    check_gj_info(b.gj2, nb * sgrp, st2, "scan forward sweep (Schur inverses)");
     const double tw2 = wall();
     if (seg_t) { seg_t[0] = tw1 - tw0; seg_t[1] = tw2 - tw1; }
     // ---------------- K matrix (KMatrixExtractor.cpp:75-198) ----------------
@@ -1297,9 +1319,12 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
             constexpr int kAhead = 4;
             int n_enq = n0;
             auto enqueue_sinv_to = [&](int n_hi) {
-                for (; n_enq < std::min(n_hi, n1); ++n_enq) {
-                    sinv_step(n_enq, b.segS.p + static_cast<size_t>(n_enq - n0) * nb * pp, bd_seg(n_enq));
-                    SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n_enq - n0], st2));
+                while (n_enq < std::min(n_hi, n1)) {   // whole groups of up to sgrp points
+                    const int a = n_enq, e = std::min(a + sgrp, n1);
+                    for (int n = a; n < e; ++n) sinv_build(n, b.segS.p + static_cast<size_t>(n - n0) * nb * pp, bd_seg(n));
+                    sinv_invert(b.segS.p + static_cast<size_t>(a - n0) * nb * pp, e - a);
+                    for (int n = a; n < e; ++n) SEHF_HIP_CHECK(hipEventRecord(g.ev_seg[n - n0], st2));
+                    n_enq = e;
                 }
             };
             enqueue_sinv_to(n0 + kAhead);
@@ -1412,7 +1437,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
         SEHF_HIP_CHECK(hipStreamSynchronize(st2));
         SEHF_HIP_CHECK(hipStreamSynchronize(st));
         check_gj_info(b.gj, nb, st, "segment cycle");
-        check_gj_info(b.gj2, nb, st2, "segment cycle (Schur inverses)");
+        check_gj_info(b.gj2, nb * sgrp, st2, "segment cycle (Schur inverses)");
         const double tb1 = wall();
         if (seg_t) { seg_t[2] = t_rec - tb0; seg_t[3] = tb1 - t_rec; }
         return;
@@ -1445,7 +1470,7 @@ void run_batch(sehf_scan* s, Group& g, const double* ks, const double* dir, int 
     }
     io_drain();
     check_gj_info(b.gj, nb, st, "scan back-propagation");
-    check_gj_info(b.gj2, nb, st2, "scan back-propagation (Schur inverses)");
+    check_gj_info(b.gj2, nb * sgrp, st2, "scan back-propagation (Schur inverses)");
     const double tw4 = wall();
     // ---------------- asymptotic fit ----------------
     if (tan) fit_dsolve(b.dAm.p, b.dBm.p, b.YJ.p, b.YY.p, b.dYJ.p, b.dYY.p, b.coef.p, b.dcoef.p, Np, nb, st);
