@@ -895,6 +895,41 @@ __global__ __launch_bounds__(256) void k_gj_final(const double* mat, double* out
     }
 }
 
+// Symmetrised form over tile pairs: the workgroup of upper-triangle tile pair
+// (I, J), I <= J, gathers G1 = X_rm(I rows, perm[J cols]) and G2 = X_rm(J
+// rows, perm[I cols]) once and writes both out(I, J) = 1/2 (G1 + G2^T) and
+// out(J, I) = 1/2 (G2 + G1^T): half the gathered reads of one tile per
+// workgroup, and the same sums in the same order.
+__global__ __launch_bounds__(256) void k_gj_final_sym(const double* mat, double* out, int64_t ld,
+                                                      int64_t stride, int n, const int* perm) {
+    const int64_t b = blockIdx.z;
+    const double* X = mat + b * stride;
+    double* O = out + b * stride;
+    const int* pm = perm + b * n;
+    __shared__ double g1[PT][PT + 1];
+    __shared__ double g2[PT][PT + 1];
+    const int t = blockIdx.x;   // t = J (J + 1) / 2 + I
+    int J = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((J + 1) * (J + 2) / 2 <= t) ++J;
+    while (J * (J + 1) / 2 > t) --J;
+    const int I = t - J * (J + 1) / 2;
+    const int i0 = I * PT, j0 = J * PT;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int yy = ty; yy < PT; yy += 8) {
+        const int i = i0 + yy, j = j0 + tx;
+        if (i < n && j < n) g1[yy][tx] = X[(int64_t)i * ld + pm[j]];
+        const int jr = j0 + yy, ic = i0 + tx;
+        if (jr < n && ic < n) g2[yy][tx] = X[(int64_t)jr * ld + pm[ic]];
+    }
+    __syncthreads();
+    for (int yy = ty; yy < PT; yy += 8) {
+        const int i = i0 + yy, j = j0 + tx;
+        if (i < n && j < n) O[(int64_t)i * ld + j] = 0.5 * (g1[yy][tx] + g2[tx][yy]);
+        const int jr = j0 + yy, ic = i0 + tx;
+        if (I != J && jr < n && ic < n) O[(int64_t)jr * ld + ic] = 0.5 * (g2[yy][tx] + g1[tx][yy]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Gauss-Jordan leaf in one workgroup per matrix, with partial pivoting
 // (orders up to WG_THREADS * R).  Thread t owns rows t + q * WG_THREADS, q < R,
@@ -1344,8 +1379,13 @@ void gj_inverse(double* mat, double* out, int n, int64_t ld, int64_t stride, int
     const int staged = (n <= 8192) ? 1 : 0;
     hipLaunchKernelGGL(k_gj_perm, dim3(batch), dim3(256), (staged ? 2 : 1) * sizeof(int) * n, s, W.piv, W.perm, n,
                        staged);
-    dim3 fg((n + PT - 1) / PT, (n + PT - 1) / PT, batch);
-    hipLaunchKernelGGL(k_gj_final, fg, dim3(256), 0, s, mat, out, ld, stride, n, W.perm, symmetrize ? 1 : 0);
+    const int nt = (n + PT - 1) / PT;
+    if (symmetrize) {
+        hipLaunchKernelGGL(k_gj_final_sym, dim3(nt * (nt + 1) / 2, 1, batch), dim3(256), 0, s, mat, out, ld, stride, n,
+                           W.perm);
+    } else {
+        hipLaunchKernelGGL(k_gj_final, dim3(nt, nt, batch), dim3(256), 0, s, mat, out, ld, stride, n, W.perm, 0);
+    }
 }
 
 // ---------------------------------------------------------------------------
