@@ -417,7 +417,8 @@ int  sehf_scan_create_with(sehf_ctx* ctx, const sehf_problem* p, int tables, seh
  * device); takes effect when the batch buffers are next allocated. */
 int  sehf_scan_set_checkpoints(sehf_scan* s, int where);
 /* Device bytes the handle holds: [0] energy-independent tables and inputs,
- * [1] batch buffers of all groups, [2] pinned host bytes (checkpoints). */
+ * [1] batch buffers of all groups, [2] pinned host bytes (checkpoints, and
+ * the staging ring of the psi files once a psi store has been used). */
 int  sehf_scan_memory(const sehf_scan* s, size_t* bytes);
 
 /* Kernel timing: between begin and end, every `every`-th FP64 GEMM and

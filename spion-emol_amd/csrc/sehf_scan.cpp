@@ -1675,6 +1675,7 @@ int sehf_scan_memory(const sehf_scan* s, size_t* bytes) {
                 (arr.host ? host : dev) += arr.bytes();
             });
             dev += gj_buffer_bytes(b.gj.n, b.gj.batch) + gj_buffer_bytes(b.gj2.n, b.gj2.batch);
+            host += g->io_stage.bytes();   // pinned staging of the psi files
         }
         bytes[0] = sh; bytes[1] = dev; bytes[2] = host;
     });
