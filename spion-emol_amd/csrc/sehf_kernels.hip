@@ -851,15 +851,18 @@ __global__ __launch_bounds__(LEAF_BLK) void k_leaf_apply(double* mat, int64_t ld
 }
 
 // Copy the block rows K (nk rows from k0) x columns J (nj from j0) of the
-// row-major view into T (row-major nk x nj).
+// row-major view into T (row-major nk x nj): one element per thread over the
+// flattened nk * nj block, so T is written contiguously and a row of M is
+// read contiguously whatever nj is.
 __global__ void k_copy_block(const double* mat, int64_t ld, int64_t stride, int k0, int nk, int j0,
                              int nj, double* tb, int64_t tstride) {
-    const int64_t b = blockIdx.z;
+    const int64_t b = blockIdx.y;
     const double* M = mat + b * stride;
     double* T = tb + b * tstride;
-    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kk = blockIdx.y;
-    if (jj < nj) T[(int64_t)kk * nj + jj] = M[(int64_t)(k0 + kk) * ld + j0 + jj];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nk * nj) return;
+    const int kk = static_cast<int>(e / nj), jj = static_cast<int>(e % nj);
+    T[e] = M[(int64_t)(k0 + kk) * ld + j0 + jj];
 }
 
 // Column-interchange composition: perm = S_{n-1} ... S_0 applied to identity.
@@ -1366,7 +1369,8 @@ static void gj_leaf(double* mat, int n, int64_t ld, int64_t stride, int batch, i
 static void gj_update(double* mat, int n, int64_t ld, int64_t stride, int batch, int k0, int nk,
                       int j0, int nj, const GjWork& W, hipStream_t s) {
     const int64_t ts = static_cast<int64_t>(gj_tbuf_elems(n));
-    dim3 cg((nj + 255) / 256, nk, batch);
+    const int64_t ne = static_cast<int64_t>(nk) * nj;
+    dim3 cg(static_cast<unsigned>((ne + 255) / 256), batch);
     hipLaunchKernelGGL(k_copy_block, cg, dim3(256), 0, s, mat, ld, stride, k0, nk, j0, nj, W.tbuf, ts);
     Gemm g;
     g.M = nj; g.N = n; g.K = nk;

@@ -10,20 +10,26 @@ namespace sehf {
 // U = 12 W^{-1} - 10 I - R_prev, except the psi-f blocks (added by
 // add_psi_f_blocks / a GEMM):  psi-psi: 12 Sinv - 10 I - R;
 // f-f: diag(12 Dinv - 10) - R;  psi-f, f-psi: -R.
-__global__ void k_init_U(double* U, const double* R, const double* Sinv, const double* Dinv, int Np,
-                         int Nt, int64_t sU, int64_t sR, int64_t sS) {
+// Each thread handles IU_PER elements of a column, 256 apart (coalesced).
+constexpr int IU_PER = 4;
+__global__ __launch_bounds__(256) void k_init_U(double* U, const double* R, const double* Sinv, const double* Dinv,
+                                                int Np, int Nt, int64_t sU, int64_t sR, int64_t sS) {
     const int64_t b = blockIdx.z;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= Nt) return;
-    const int64_t o = i + (int64_t)j * Nt;
-    double v = -R[b * sR + o];
-    if (i < Np && j < Np) v += 12.0 * Sinv[b * sS + i + (int64_t)j * Np] - (i == j ? 10.0 : 0.0);
-    else if (i >= Np && j >= Np && i == j) v += 12.0 * Dinv[i - Np] - 10.0;
-    U[b * sU + o] = v;
+    const int j = blockIdx.y;
+#pragma unroll
+    for (int q = 0; q < IU_PER; ++q) {
+        const int i = blockIdx.x * (256 * IU_PER) + q * 256 + threadIdx.x;
+        if (i >= Nt) return;
+        const int64_t o = i + (int64_t)j * Nt;
+        double v = -R[b * sR + o];
+        if (i < Np && j < Np) v += 12.0 * Sinv[b * sS + i + (int64_t)j * Np] - (i == j ? 10.0 : 0.0);
+        else if (i >= Np && j >= Np && i == j) v += 12.0 * Dinv[i - Np] - 10.0;
+        U[b * sU + o] = v;
+    }
 }
 void init_U(double* U, const double* R, const double* Sinv, const double* Dinv, int Np, int Nt,
             int batch, hipStream_t s) {
-    dim3 g((Nt + 255) / 256, Nt, batch);
+    dim3 g((Nt + 256 * IU_PER - 1) / (256 * IU_PER), Nt, batch);
     hipLaunchKernelGGL(k_init_U, g, dim3(256), 0, s, U, R, Sinv, Dinv, Np, Nt, (int64_t)Nt * Nt,
                        (int64_t)Nt * Nt, (int64_t)Np * Np);
 }
