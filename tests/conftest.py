@@ -20,3 +20,15 @@ def rel_err(got, ref):
     got = np.asarray(got)
     ref = np.asarray(ref)
     return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """Device sessions: torch, which ships its own HIP runtime, initialises
+    it before libsehf's runtime is first used, the order of a host program
+    that imports torch first (smoke(), the autograd wrapper)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield
