@@ -78,13 +78,36 @@ __global__ __launch_bounds__(256) void k_spd_leaf(double* M, int64_t ld, int64_t
 }
 
 // M <- 1/2 (M + M^T) in place (n x n blocks, leading dim ld).
-__global__ void k_spd_sym(double* M, int64_t ld, int64_t stride, int n) {
+// M <- (M + M^T) / 2 over 32 x 32 tile pairs: the workgroup of pair (P, Q),
+// Q <= P, reads the lower tile (P, Q) and its mirror (Q, P) coalesced through
+// LDS and writes m(i, j) = m(j, i) = (m(i, j) + m(j, i)) / 2 for i > j.
+constexpr int SYM_T = 32;
+__global__ __launch_bounds__(256) void k_spd_sym(double* M, int64_t ld, int64_t stride, int n) {
     double* m = M + static_cast<int64_t>(blockIdx.z) * stride;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n || i <= j) return;
-    const double v = 0.5 * (m[i + static_cast<int64_t>(j) * ld] + m[j + static_cast<int64_t>(i) * ld]);
-    m[i + static_cast<int64_t>(j) * ld] = v;
-    m[j + static_cast<int64_t>(i) * ld] = v;
+    const int t = blockIdx.x;   // t = P (P + 1) / 2 + Q
+    int P = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((P + 1) * (P + 2) / 2 <= t) ++P;
+    while (P * (P + 1) / 2 > t) --P;
+    const int Q = t - P * (P + 1) / 2;
+    const int p0 = P * SYM_T, q0 = Q * SYM_T;
+    __shared__ double Lt[SYM_T][SYM_T + 1];   // Lt[jj][ii] = m(p0 + ii, q0 + jj)
+    __shared__ double Ut[SYM_T][SYM_T + 1];   // Ut[jj][ii] = m(q0 + ii, p0 + jj)
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int yy = ty; yy < SYM_T; yy += 8) {
+        if (p0 + tx < n && q0 + yy < n) Lt[yy][tx] = m[(p0 + tx) + static_cast<int64_t>(q0 + yy) * ld];
+        if (q0 + tx < n && p0 + yy < n) Ut[yy][tx] = m[(q0 + tx) + static_cast<int64_t>(p0 + yy) * ld];
+    }
+    __syncthreads();
+    for (int yy = ty; yy < SYM_T; yy += 8) {
+        {   // lower element (i, j) = (p0 + tx, q0 + yy)
+            const int i = p0 + tx, j = q0 + yy;
+            if (i < n && j < n && i > j) m[i + static_cast<int64_t>(j) * ld] = 0.5 * (Lt[yy][tx] + Ut[tx][yy]);
+        }
+        {   // upper element (j, i) = (q0 + tx, p0 + yy), mirror of (p0 + yy, q0 + tx)
+            const int i = p0 + yy, j = q0 + tx;
+            if (i < n && j < n && i > j) m[j + static_cast<int64_t>(i) * ld] = 0.5 * (Lt[tx][yy] + Ut[yy][tx]);
+        }
+    }
 }
 
 void spd_rec(double* M, int64_t ld, int64_t stride, int off, int m, int batch, double* T, int* info,
@@ -140,8 +163,8 @@ void spd_inverse(double* mat, int n, int64_t ld, int64_t stride, int batch, bool
     if (n <= 0 || batch <= 0) return;
     spd_rec(mat, ld, stride, 0, n, batch, w.tbuf, w.info, s);
     if (symmetrize) {
-        dim3 g((n + 255) / 256, n, batch);
-        hipLaunchKernelGGL(k_spd_sym, g, dim3(256), 0, s, mat, ld, stride, n);
+        const int nt = (n + SYM_T - 1) / SYM_T;
+        hipLaunchKernelGGL(k_spd_sym, dim3(nt * (nt + 1) / 2, 1, batch), dim3(256), 0, s, mat, ld, stride, n);
     }
 }
 
