@@ -88,6 +88,17 @@ PsiWriter::PsiWriter(const std::string& root, int ik0, int nb, const double* ks,
     on_[0] = tail && lay.n_tail > 0;
     on_[1] = main && lay.n_main > 0;
     e_.resize(nb);
+    try {
+        open_all(root, ik0, ks);
+    } catch (...) {
+        cleanup();   // a throwing constructor runs no destructor
+        throw;
+    }
+}
+
+void PsiWriter::open_all(const std::string& root, int ik0, const double* ks) {
+    const PsiLayout& lay = lay_;
+    const int nb = static_cast<int>(e_.size());
     const int64_t mat = static_cast<int64_t>(lay.Np) * lay.Np * 8;
     for (int e = 0; e < nb; ++e) {
         One& o = e_[e];
@@ -117,10 +128,13 @@ PsiWriter::PsiWriter(const std::string& root, int ik0, int nb, const double* ks,
     }
 }
 
-PsiWriter::~PsiWriter() {
+PsiWriter::~PsiWriter() { cleanup(); }
+
+void PsiWriter::cleanup() {
     for (One& o : e_)
         for (int f = 0; f < 2; ++f) {
             if (o.fd[f] >= 0) ::close(o.fd[f]);
+            o.fd[f] = -1;
             if (!done_ && !o.tmp[f].empty()) ::unlink(o.tmp[f].c_str());
         }
 }
@@ -161,6 +175,18 @@ void PsiWriter::commit() {
 PsiReader::PsiReader(const std::string& root, int ik0, int nb, const double* ks, const PsiLayout& lay, bool need_main)
     : lay_(lay), main_(need_main) {
     e_.resize(nb);
+    try {
+        open_all(root, ik0, ks);
+    } catch (...) {
+        close_all();   // a throwing constructor runs no destructor
+        throw;
+    }
+}
+
+void PsiReader::open_all(const std::string& root, int ik0, const double* ks) {
+    const PsiLayout& lay = lay_;
+    const bool need_main = main_;
+    const int nb = static_cast<int>(e_.size());
     for (int e = 0; e < nb; ++e) {
         const std::string dir = psi_dir(root, ik0 + e);
         errno = 0;
@@ -202,10 +228,14 @@ PsiReader::PsiReader(const std::string& root, int ik0, int nb, const double* ks,
     }
 }
 
-PsiReader::~PsiReader() {
+PsiReader::~PsiReader() { close_all(); }
+
+void PsiReader::close_all() {
     for (One& o : e_)
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < 2; ++f) {
             if (o.fd[f] >= 0) ::close(o.fd[f]);
+            o.fd[f] = -1;
+        }
 }
 
 void PsiReader::get(int n, double* batch) const {

@@ -49,6 +49,8 @@ class PsiWriter {
     void commit();
 
    private:
+    void open_all(const std::string& root, int ik0, const double* ks);
+    void cleanup();
     struct One {
         std::string dir;
         int fd[2] = {-1, -1};
@@ -74,6 +76,8 @@ class PsiReader {
     void get(int n, double* batch) const;
 
    private:
+    void open_all(const std::string& root, int ik0, const double* ks);
+    void close_all();
     struct One {
         int fd[2] = {-1, -1};
         std::string path[2];
